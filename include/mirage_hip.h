@@ -1,0 +1,245 @@
+/*
+ * mirage_hip.h - C ABI of the MI355X (gfx950) read-classification library.
+ *
+ * Drop-in for MirAge's matrix path (mirdesign/mirage): the reference's
+ * Analyser calls
+ *   vector<MatrixResult> CUDAGetHighestScore(...)      MirAge/Analyser.h:228,
+ *                                                      MirAge/MatrixGPUCU.cu:571
+ *   void   InitialiseCuda(int numDevices)              MirAge/Analyser.h:231
+ *   size_t Get_LOCAL_DATA_rowhit_SIZE()                MirAge/Analyser.h:232
+ *   size_t CUDAGetMaxBlockSize(int)                    MirAge/Hardware.h:16
+ * and its CPU sibling MatrixCPU::CPUGetHighestScore (MirAge/MatrixCPU.h:41).
+ * The C++ shim in mirage_amd/csrc/matrix_hip.hpp keeps that call surface and
+ * returns vector<MatrixResult>; this header is the plain-pointer layer below
+ * it.  Unlike the reference GPU form, the whole path runs on the device:
+ * k-mer probe, candidate filtering, diagonal scoring, best hit, strand merge.
+ *
+ * Conventions: every entry point returns mh_status (0 = success, negative =
+ * error; no exceptions cross this boundary, cf. MirAge/MatrixGPUCU.cu:490-530
+ * which throws).  "d_" arguments are device pointers, "h_" host pointers.
+ * A handle owns one HIP stream; calls on one handle are serialised by the
+ * caller (as mutexCUDARunning does, MirAge/Analyser.cpp:718,989); different
+ * handles may run concurrently.
+ */
+#ifndef MIRAGE_HIP_H
+#define MIRAGE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t mh_status;
+#define MH_OK 0
+#define MH_ERR_ARG (-1)        /* bad argument */
+#define MH_ERR_HIP (-2)        /* HIP runtime error (message via mh_last_error) */
+#define MH_ERR_NOMEM (-3)      /* device allocation failed (caller may shrink the batch) */
+#define MH_ERR_RANGE (-4)      /* read longer than the handle's maximum */
+#define MH_ERR_OVERFLOW (-5)   /* an output pool was too small; *_needed says how big */
+#define MH_ERR_BADBASE (-6)    /* a read holds a byte with no complement */
+#define MH_ERR_STATE (-7)      /* wrong order of calls */
+
+typedef struct mh_handle mh_handle;
+typedef struct mh_index mh_index;
+
+/* Index-build flags. */
+#define MH_INDEX_LAST_BUCKET_EMPTY 1u /* keep the reference loader's behaviour:
+                                         the highest-hash bucket is present but
+                                         empty (MirAge/Hashmap.cpp:540-554) */
+
+/* Classification flags. */
+#define MH_CLASSIFY_ULTRAFAST 1u   /* no head/tail extension (MatrixCPU.cpp:69) */
+#define MH_CLASSIFY_NO_DIRECT 2u   /* disable the direct-hit fast path */
+#define MH_CLASSIFY_BATCHED 4u     /* length-bucketed persistent kernels */
+
+/* Per-read result after the strand merge (MirAge/Analyser.cpp:592-611).
+ * Top-tie reference ids (ascending) are top_pool[top_off, top_off+n_top).
+ * n_top is 0 when the reference would print nothing for the read
+ * (MirAge/Analyser.cpp:388). */
+typedef struct {
+  float highest;     /* HighestScore of the surviving strand, -1 if none */
+  uint32_t strand;   /* 0 forward, 1 reverse complement */
+  uint32_t n_top;
+  uint32_t top_off;
+  uint32_t n_cand;   /* candidates with Score > -1 on the surviving strand */
+  uint32_t status;   /* 0, or MH_READ_* bits */
+} mh_read_result;
+
+#define MH_READ_BADBASE 1u
+#define MH_READ_TOO_LONG 2u
+#define MH_READ_CAPACITY 4u  /* more bucket entries than the largest workspace */
+
+/* One seeded candidate (detail output for parity checking). */
+typedef struct {
+  uint32_t read;
+  uint32_t strand;
+  uint32_t ref;
+  uint32_t hits;     /* popcount of the row-hit bitmap */
+  uint64_t sum_r;    /* diagonalRepeatedTotalScore (MatrixCPU.cpp:183) */
+  uint64_t sum_c;    /* diagonalTotalConcidered (MatrixCPU.cpp:181) */
+  float score;       /* MatrixCPU Score, -2 when no seed reached T */
+  float rel;         /* MatrixResult::GetRelativeScore, -1 when score <= -1 */
+  uint32_t n_seeds;
+  uint32_t pad;
+} mh_cand;
+
+typedef struct {
+  uint64_t n_entries;      /* (ref, pos) entries in HBM */
+  uint64_t n_buckets;      /* non-empty buckets */
+  uint64_t device_bytes;   /* HBM held by the index */
+  uint32_t n_refs;
+  int32_t threshold;       /* T; stride T/2, k-mer length T - T/2 + 1 */
+  uint32_t empty_hash;     /* present-but-empty bucket, 0xFFFFFFFF if none */
+  uint32_t pad;
+} mh_index_info;
+
+/* Work counters of the last classify call on a handle. */
+typedef struct {
+  uint64_t queries;        /* read strands */
+  uint64_t probes;         /* k-mer lookups */
+  uint64_t hits;           /* lookups that found a bucket */
+  uint64_t entries;        /* bucket entries gathered */
+  uint64_t seeds;          /* seeds scored */
+  uint64_t cands;          /* seeded candidates */
+  uint64_t direct;         /* strands resolved by the direct-hit fast path */
+  uint64_t overflow;       /* strands sent to the large workspace */
+} mh_counters;
+
+/* ---- handle ------------------------------------------------------------ */
+
+/* Opens device `device` (replaces InitialiseCuda, MatrixGPUCU.cu:481-488).
+ * max_read_len bounds read length (replaces Get_LOCAL_DATA_rowhit_SIZE,
+ * MatrixGPUCU.cu:39-42); 0 selects 32768. */
+mh_status mh_open(int32_t device, uint32_t max_read_len, mh_handle **out);
+mh_status mh_close(mh_handle *h);
+const char *mh_last_error(const mh_handle *h);
+/* The handle's HIP stream (a hipStream_t). */
+void *mh_stream(mh_handle *h);
+/* Largest read length the handle accepts (Get_LOCAL_DATA_rowhit_SIZE). */
+uint32_t mh_max_read_len(const mh_handle *h);
+/* Preferred reads per batch for the device (CUDAGetMaxBlockSize,
+ * MatrixGPUCU.cu:549-556). */
+uint64_t mh_max_batch_reads(const mh_handle *h);
+
+/* ---- index ------------------------------------------------------------- */
+
+/* Builds the bucketed hash index on the device from host reference bytes
+ * (ref_off: n_refs+1 byte offsets).  Bucket contents follow
+ * Hashmap::TBuildHashmapForReference (MirAge/Hashmap.cpp:2771-2976) and
+ * SaveHashmapTableV3's fragment rule (:1759-1833). */
+mh_status mh_index_build(mh_handle *h, const uint8_t *h_ref_bytes, const uint64_t *h_ref_off,
+                         uint32_t n_refs, int32_t threshold, uint32_t flags, mh_index **out);
+mh_status mh_index_free(mh_index *idx);
+mh_status mh_index_get_info(const mh_index *idx, mh_index_info *out);
+/* Copies the entries of bucket `hash` (sorted by ref, pos) to host arrays.
+ * *count receives the bucket size, or -1 when the hash is absent. */
+mh_status mh_index_bucket(mh_handle *h, const mh_index *idx, uint32_t hash, uint32_t *h_refs,
+                          uint32_t *h_pos, uint64_t cap, int64_t *count);
+
+/* ---- classification ---------------------------------------------------- */
+
+/* Full path on device-resident reads: d_reads bytes, d_read_off n_reads+1
+ * offsets.  Results go to d_out[n_reads] and the top-tie pool d_top_pool
+ * (capacity top_cap ids).  d_cands (may be NULL) receives every seeded
+ * candidate of both strands, in no particular order.  On MH_ERR_OVERFLOW,
+ * *top_needed / *cand_needed hold the sizes required.  Asynchronous on the
+ * handle's stream unless h_n_top/h_n_cand are non-NULL, which makes the call
+ * wait for completion and report the pool fill. */
+mh_status mh_classify(mh_handle *h, const mh_index *idx, const uint8_t *d_reads,
+                      const uint64_t *d_read_off, uint32_t n_reads, uint32_t flags,
+                      mh_read_result *d_out, uint32_t *d_top_pool, uint64_t top_cap,
+                      mh_cand *d_cands, uint64_t cand_cap, uint64_t *top_needed,
+                      uint64_t *cand_needed);
+
+/* Same path from host memory: uploads reads, classifies, downloads.
+ * h_top_pool must hold top_cap ids; h_cands may be NULL. */
+mh_status mh_classify_host(mh_handle *h, const mh_index *idx, const uint8_t *h_reads,
+                           const uint64_t *h_read_off, uint32_t n_reads, uint32_t flags,
+                           mh_read_result *h_out, uint32_t *h_top_pool, uint64_t top_cap,
+                           mh_cand *h_cands, uint64_t cand_cap, uint64_t *top_needed,
+                           uint64_t *cand_needed);
+
+/* Counters of the last mh_classify / mh_classify_host call (synchronises). */
+mh_status mh_get_counters(mh_handle *h, mh_counters *out);
+
+/* Scoring only, the role CUDAGetHighestScore plays in the reference
+ * (MatrixGPUCU.cu:108-467 with host :571-1969): score given seeds.
+ * Pairs p = 0..n_pairs-1 score query pair_query[p] (bytes
+ * d_queries[d_query_off[q], d_query_off[q+1])) against reference
+ * pair_ref[p] of idx, with seeds d_seeds[4*d_seed_off[p] .. 4*d_seed_off[p+1])
+ * as (start.ref, start.query, last.ref, last.query) - the
+ * CombiPositions / CombiPositions_LastPositionInRange pairs of
+ * Analyser.h:97-98.  Outputs per pair: Score (MatrixCPU.cpp:195-213),
+ * relative score (MatrixResult.cpp:21-29, -1 when Score <= -1), and the
+ * integer triple. */
+mh_status mh_score_seeds(mh_handle *h, const mh_index *idx, const uint8_t *d_queries,
+                         const uint64_t *d_query_off, const uint32_t *d_pair_query,
+                         const uint32_t *d_pair_ref, const uint64_t *d_seed_off,
+                         const uint32_t *d_seeds, uint32_t n_pairs, uint32_t flags,
+                         float *d_score, float *d_rel, uint64_t *d_sum_r, uint64_t *d_sum_c,
+                         uint32_t *d_hits);
+
+/* ---- batched form and tuning -------------------------------------------- */
+
+/* Tile choice for the length-bucketed persistent form. */
+typedef struct {
+  uint32_t max_len;        /* bucket holds reads with length <= max_len */
+  uint32_t block;          /* threads per workgroup: 64, 256 or 1024 */
+  uint32_t entries_cap;    /* LDS capacity in bucket entries (power of two) */
+  uint32_t grid;           /* persistent workgroups (0: derive from occupancy) */
+} mh_tile;
+
+/* Installs the bucket table used by MH_CLASSIFY_BATCHED (at most 8 rows,
+ * ascending max_len; the last row must cover mh_max_read_len). */
+mh_status mh_set_tiles(mh_handle *h, const mh_tile *tiles, uint32_t n_tiles);
+mh_status mh_get_tiles(const mh_handle *h, mh_tile *tiles, uint32_t cap, uint32_t *n_tiles);
+/* Times candidate tiles per length bucket on the given device-resident reads
+ * and installs the fastest (the project's autotuner). */
+mh_status mh_autotune(mh_handle *h, const mh_index *idx, const uint8_t *d_reads,
+                      const uint64_t *d_read_off, uint32_t n_reads, uint32_t iters);
+
+/* ---- serving ------------------------------------------------------------ */
+
+typedef struct mh_server mh_server;
+
+typedef struct {
+  uint64_t hbm_budget;      /* bytes of HBM for requests in flight (0: 8 GiB) */
+  uint32_t max_inflight;    /* batches on the device at once (0: 3) */
+  uint32_t max_batch_reads; /* reads per coalesced batch (0: 262144) */
+  uint32_t flags;           /* classification flags for every batch */
+  uint32_t pad;
+} mh_server_config;
+
+/* Completed request. */
+typedef struct {
+  uint64_t request_id;
+  uint32_t n_reads;
+  int32_t status;
+  double submit_s;          /* monotonic seconds */
+  double admit_s;
+  double done_s;
+} mh_request_info;
+
+/* Starts a serving loop over a resident index: requests are admitted as HBM
+ * budget frees up, coalesced into batches, and several batches run on the
+ * device at once (continuous batching). */
+mh_status mh_server_start(mh_handle *h, const mh_index *idx, const mh_server_config *cfg,
+                          mh_server **out);
+/* Submits a request (host reads are copied; the call does not block on the
+ * device).  *request_id receives its id. */
+mh_status mh_server_submit(mh_server *s, const uint8_t *h_reads, const uint64_t *h_read_off,
+                           uint32_t n_reads, uint64_t *request_id);
+/* Waits for request_id (timeout_s < 0: forever) and copies its results:
+ * h_out[n_reads], top-tie ids to h_top (capacity top_cap; *top_needed
+ * receives the count, MH_ERR_OVERFLOW when larger). */
+mh_status mh_server_wait(mh_server *s, uint64_t request_id, double timeout_s,
+                         mh_read_result *h_out, uint32_t *h_top, uint64_t top_cap,
+                         uint64_t *top_needed, mh_request_info *info);
+mh_status mh_server_stop(mh_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MIRAGE_HIP_H */

@@ -1,0 +1,1165 @@
+// Read classification on the device: k-mer probe, successive-occurrence
+// filter, diagonal chains, seed scoring with head/tail extension, best hit
+// and top-tie set, strand merge.
+//
+// One workgroup processes one query (one strand of one read) with its whole
+// working set in LDS: the query bases, the probe hits, the gathered bucket
+// entries and the row-hit bitmap.  Reference semantics, step by step:
+//   probe        Analyser.cpp:1453-1520  every s-th window, j < m - T
+//   successive   Analyser.cpp:2004-2197  longest run of consecutive hit
+//                                        indices per reference; keep refs
+//                                        whose run reaches max(1, M - 1)
+//   sort/group   Analyser.cpp:2214-2315, :1024-1059  (ref, refPos, qPos)
+//   chains       Analyser.cpp:2329-2490  links step s on both axes; only
+//                                        refs whose longest chain equals the
+//                                        query maximum G >= 2 get seeds
+//   score        MatrixCPU.cpp:18-214    tail on the diagonal, head one
+//                                        base off it, bitmap popcount
+//   best hit     MatrixCPU.cpp:323-370, MatrixResult.cpp:21-29
+//   strand merge Analyser.cpp:592-611
+// The run and chain steps are done with LDS bitonic sorts and block scans
+// instead of the reference's per-reference dense arrays; the head and tail
+// comparisons of one seed are spread over the 64 lanes of a wavefront and
+// resolved with a ballot.
+//
+// Queries whose working set does not fit the workgroup's LDS are appended
+// to an overflow list and finished by a 1024-thread kernel whose workspace
+// lives in HBM.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "mh_internal.hpp"
+
+namespace mh {
+namespace {
+
+__constant__ uint8_t kComp[256];
+
+__device__ __forceinline__ uint8_t upper(uint8_t c) {
+  return (int8_t)c > 96 ? (uint8_t)(c - 32) : c;  // Sequences.cpp:43-52
+}
+
+__device__ __forceinline__ uint32_t hash28(const uint8_t *p, int k) {
+  uint32_t h = 2166136261u;  // Hashmap.cpp:3197-3211
+  for (int i = 0; i < k; i++) h = (h * 16777619u) ^ p[i];
+  return ((h >> kHashBits) ^ h) & kHashMask;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v = max(v, (uint32_t)__shfl_xor(v, d, 64));
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += (uint32_t)__shfl_xor(v, d, 64);
+  return v;
+}
+
+// Exclusive block scan; red holds at least BS/64 words.  All threads get the
+// block total.  Contains barriers when BS > 64.
+template <int BS>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *red, uint32_t &total) {
+  constexpr int NW = BS / 64;
+  const uint32_t inc = wave_incl_scan(v);
+  if constexpr (NW == 1) {
+    total = __shfl(inc, 63, 64);
+    return inc - v;
+  } else {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 63) red[w] = inc;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; i++) {
+      uint32_t x = red[i];
+      off += (i < w) ? x : 0u;
+      tot += x;
+    }
+    total = tot;
+    return off + inc - v;
+  }
+}
+
+template <int BS>
+__device__ __forceinline__ uint32_t block_max(uint32_t v, uint32_t *red) {
+  constexpr int NW = BS / 64;
+  v = wave_max(v);
+  if constexpr (NW == 1) {
+    return v;
+  } else {
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t r = 0;
+#pragma unroll
+    for (int i = 0; i < NW; i++) r = max(r, red[i]);
+    return r;
+  }
+}
+
+template <int BS>
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *red) {
+  constexpr int NW = BS / 64;
+  v = wave_sum(v);
+  if constexpr (NW == 1) {
+    return v;
+  } else {
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t r = 0;
+#pragma unroll
+    for (int i = 0; i < NW; i++) r += red[i];
+    return r;
+  }
+}
+
+__device__ __forceinline__ bool pair_less(uint64_t ka, uint32_t va, uint64_t kb, uint32_t vb) {
+  return ka < kb || (ka == kb && va < vb);
+}
+
+// Ascending bitonic sort of (key, val) pairs over n (power of two) slots.
+template <int BS>
+__device__ void bitonic_sort(uint64_t *key, uint32_t *val, uint32_t n) {
+  for (uint32_t size = 2; size <= n; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      __syncthreads();
+      for (uint32_t t = threadIdx.x; t < (n >> 1); t += BS) {
+        const uint32_t i = 2 * stride * (t / stride) + (t % stride);
+        const uint32_t j = i + stride;
+        const bool asc = (i & size) == 0;
+        const uint64_t ka = key[i], kb = key[j];
+        const uint32_t va = val[i], vb = val[j];
+        const bool sw = asc ? pair_less(kb, vb, ka, va) : pair_less(ka, va, kb, vb);
+        if (sw) {
+          key[i] = kb;
+          key[j] = ka;
+          val[i] = vb;
+          val[j] = va;
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t next_pow2(uint32_t x) {
+  return x <= 1 ? 1u : 1u << (32 - __clz(x - 1));
+}
+
+// Workspace carved from LDS (or HBM for the large kernel).
+struct Ws {
+  uint8_t *q;          // qcap query bases
+  uint32_t *hit_j;     // pcap
+  uint64_t *hit_lo;    // pcap
+  uint32_t *hit_base;  // pcap + 1
+  uint64_t *k1;        // ecap
+  uint32_t *v1;        // ecap
+  uint64_t *k2;        // ecap
+  uint32_t *v2;        // ecap
+  uint32_t *t1, *t2, *t3, *rref;  // ecap each
+  uint32_t *bitmap;    // qcap / 32 + 1
+  uint32_t *red;       // 64 words
+  unsigned long long *acc;  // 2
+};
+
+__host__ __device__ inline size_t al8(size_t x) { return (x + 7) & ~size_t(7); }
+
+__host__ __device__ inline size_t ws_bytes(const Caps &c) {
+  size_t b = 0;
+  b += al8(c.qcap);
+  b += al8(c.pcap * 4ull) + al8(c.pcap * 8ull) + al8((c.pcap + 1) * 4ull);
+  b += 2 * (al8(c.ecap * 8ull) + al8(c.ecap * 4ull));
+  b += 4 * al8(c.ecap * 4ull);
+  b += al8((c.qcap / 32 + 1) * 4ull);
+  b += 64 * 4 + 16;
+  return b;
+}
+
+__device__ inline Ws ws_carve(unsigned char *base, const Caps &c) {
+  Ws w;
+  size_t o = 0;
+  auto take = [&](size_t n) {
+    unsigned char *p = base + o;
+    o += al8(n);
+    return p;
+  };
+  w.q = take(c.qcap);
+  w.hit_j = (uint32_t *)take(c.pcap * 4ull);
+  w.hit_lo = (uint64_t *)take(c.pcap * 8ull);
+  w.hit_base = (uint32_t *)take((c.pcap + 1) * 4ull);
+  w.k1 = (uint64_t *)take(c.ecap * 8ull);
+  w.v1 = (uint32_t *)take(c.ecap * 4ull);
+  w.k2 = (uint64_t *)take(c.ecap * 8ull);
+  w.v2 = (uint32_t *)take(c.ecap * 4ull);
+  w.t1 = (uint32_t *)take(c.ecap * 4ull);
+  w.t2 = (uint32_t *)take(c.ecap * 4ull);
+  w.t3 = (uint32_t *)take(c.ecap * 4ull);
+  w.rref = (uint32_t *)take(c.ecap * 4ull);
+  w.bitmap = (uint32_t *)take((c.qcap / 32 + 1) * 4ull);
+  w.red = (uint32_t *)take(64 * 4);
+  w.acc = (unsigned long long *)take(16);
+  return w;
+}
+
+struct Ctx {
+  IndexView ix;
+  Luts lut;
+  Reads rd;
+  uint32_t flags;
+  Outputs out;
+};
+
+enum : uint32_t { kDone = 0, kOverflow = 1 };
+
+__device__ __forceinline__ float f32_from_bits(uint32_t u) { return __uint_as_float(u); }
+
+// Relative score (MatrixResult::GetRelativeScore, MatrixResult.cpp:21-29)
+// from the logf tables; returns false when the value is outside the tables.
+__device__ __forceinline__ bool rel_score(const Luts &lut, uint32_t m, uint64_t ref_len,
+                                          float score, float &rel) {
+  const float x = score + 1.0f;
+  const uint32_t xi = __float_as_uint(x) - 0x3f800000u;
+  const uint64_t L = (uint64_t)m < ref_len ? (uint64_t)m : ref_len;
+  if (xi >= lut.n_x || L >= lut.n_smax) return false;
+  rel = lut.x_log[xi] / lut.smax_log[L];
+  return true;
+}
+
+// Score = (sumR / sumC) / 100 * percentageMatch (MatrixCPU.cpp:195-213).
+__device__ __forceinline__ float matrix_score(uint64_t sum_r, uint64_t sum_c, uint32_t hits,
+                                              uint32_t m) {
+  if (sum_c == 0) return -2.0f;
+  const float sf = (float)sum_r / (float)sum_c;
+  const float pct = (float)((100.0 / (double)(float)m) * (double)(float)hits);
+  return (float)(((double)sf / 100.0) * (double)pct);
+}
+
+// Head/tail extension of one seed by one wavefront (MatrixCPU.cpp:93-172).
+// Returns the matched length tot and the head length h; all lanes agree.
+__device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const uint8_t *rb,
+                                            uint64_t ref_len, uint32_t sref, uint32_t rs,
+                                            uint32_t lref, uint32_t re, int T, bool ultrafast,
+                                            uint32_t &tot, uint32_t &head) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t mid = re - rs;
+  uint32_t t = 0, h = 0;
+  if (!ultrafast) {
+    // Tail: at most min(T, ref_len - lref) bases, inside the query.
+    uint64_t tl = (uint64_t)lref < ref_len ? ref_len - lref : 0;
+    tl = tl < (uint64_t)T ? tl : (uint64_t)T;
+    const uint64_t qroom = re < m ? (uint64_t)(m - re) : 0;
+    tl = tl < qroom ? tl : qroom;
+    t = (uint32_t)tl;
+    for (uint32_t b = 0; b < tl; b += 64) {
+      const uint32_t l = b + lane;
+      const bool mm = l < tl && q[re + l] != rb[(uint64_t)lref + l];
+      const unsigned long long bal = __ballot(mm);
+      if (bal) {
+        t = b + (uint32_t)__ffsll((long long)bal) - 1;
+        break;
+      }
+    }
+    // Head: query row rs-1-l against reference byte sref-l; the base that
+    // meets a boundary is not counted.
+    if (rs > 0) {
+      const uint32_t hs = sref < (uint32_t)T ? sref : (uint32_t)T;
+      if (hs > 0) {
+        const uint32_t lb = (rs - 1) < (hs - 1) ? (rs - 1) : (hs - 1);
+        h = lb;
+        for (uint32_t b = 0; b <= lb; b += 64) {
+          const uint32_t l = b + lane;
+          const bool mm = l <= lb && q[rs - 1 - l] != rb[sref - l];
+          const unsigned long long bal = __ballot(mm);
+          if (bal) {
+            h = b + (uint32_t)__ffsll((long long)bal) - 1;
+            break;
+          }
+        }
+      }
+    }
+  }
+  tot = t + h + mid;
+  head = h;
+}
+
+__device__ __forceinline__ void set_bits(uint32_t *bm, uint32_t a, uint32_t len) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t b = a + len;  // [a, b)
+  const uint32_t w0 = a >> 5, w1 = (b - 1) >> 5;
+  for (uint32_t w = w0 + lane; w <= w1; w += 64) {
+    uint32_t mask = 0xFFFFFFFFu;
+    if (w == w0) mask &= 0xFFFFFFFFu << (a & 31);
+    if (w == w1) mask &= 0xFFFFFFFFu >> (31 - ((b - 1) & 31));
+    atomicOr(&bm[w], mask);
+  }
+}
+
+// Scores one reference from its seeds: every wave takes seeds in turn.
+// get_seed(i, sref, sq, lref, lq) supplies seed i.  Returns on all threads.
+template <int BS, class GetSeed>
+__device__ void score_ref(const uint8_t *q, uint32_t m, const uint8_t *rb, uint64_t ref_len,
+                          uint32_t n_seeds, int T, bool ultrafast, const Ws &ws,
+                          GetSeed get_seed, uint64_t &sum_r, uint64_t &sum_c, uint32_t &hits) {
+  constexpr uint32_t NW = BS / 64;
+  const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t nw = (m + 31) >> 5;
+  __syncthreads();
+  for (uint32_t w = threadIdx.x; w < nw; w += BS) ws.bitmap[w] = 0;
+  if (threadIdx.x == 0) {
+    ws.acc[0] = 0;
+    ws.acc[1] = 0;
+  }
+  __syncthreads();
+  unsigned long long sr = 0, sc = 0;
+  for (uint32_t i = wid; i < n_seeds; i += NW) {
+    uint32_t sref, sq, lref, lq;
+    get_seed(i, sref, sq, lref, lq);
+    const uint32_t rs = sq, re = lq;
+    if (rs > re) continue;  // MatrixCPU.cpp:57-58
+    uint32_t tot, h;
+    extend_seed(q, m, rb, ref_len, sref, rs, lref, re, T, ultrafast, tot, h);
+    if (tot >= (uint32_t)T) {  // MatrixCPU.cpp:178-184
+      sc += tot;
+      sr += (unsigned long long)tot * (tot + 1) / 2;
+      set_bits(ws.bitmap, rs - h, tot);
+    }
+  }
+  if (lane == 0 && sc) {
+    atomicAdd(&ws.acc[0], sr);
+    atomicAdd(&ws.acc[1], sc);
+  }
+  __syncthreads();
+  uint32_t pc = 0;
+  for (uint32_t w = threadIdx.x; w < nw; w += BS) pc += __popc(ws.bitmap[w]);
+  hits = block_sum<BS>(pc, ws.red);
+  __syncthreads();
+  sum_r = ws.acc[0];
+  sum_c = ws.acc[1];
+}
+
+__device__ __forceinline__ void write_query(const Ctx &c, uint32_t q, const QueryOut &r) {
+  if (threadIdx.x == 0) c.out.q[q] = r;
+}
+
+// Processes query q.  Returns kOverflow (uniformly) when the working set
+// exceeds the workspace capacities; nothing is written in that case.
+template <int BS>
+__device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, uint32_t q,
+                                  bool last_resort) {
+  constexpr uint32_t NW = BS / 64;
+  (void)NW;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t read = q >> 1, strand = q & 1;
+  const IndexView &ix = c.ix;
+  const int T = ix.T, s = ix.s, k = ix.k;
+  const uint64_t r0 = c.rd.off[read];
+  const uint64_t m64 = c.rd.off[read + 1] - r0;
+  QueryOut res;
+  res.highest = -1.0f;
+  res.n_top = 0;
+  res.top_off = 0;
+  res.n_cand = 0;
+  res.status = 0;
+  if (m64 > cap.qcap) {
+    if (!last_resort) return kOverflow;
+    res.status = MH_READ_TOO_LONG;
+    write_query(c, q, res);
+    return kDone;
+  }
+  const uint32_t m = (uint32_t)m64;
+
+  // Stage the strand: uppercase, reverse complement for strand 1
+  // (Sequences.cpp:43-52, :795-841).
+  int bad = 0;
+  for (uint32_t x = tid; x < m; x += BS) {
+    uint8_t ch = upper(c.rd.bytes[r0 + (strand ? (m - 1 - x) : x)]);
+    if (strand) ch = kComp[ch];
+    bad |= (ch == 0);
+    ws.q[x] = ch;
+  }
+  bad = __syncthreads_or(bad);
+  if (bad) {
+    res.status = MH_READ_BADBASE;
+    write_query(c, q, res);
+    return kDone;
+  }
+  if (m <= (uint32_t)T) {  // no well-defined probe (the reference reads past the end)
+    write_query(c, q, res);
+    return kDone;
+  }
+  const uint32_t nprobe = (m - T + s - 1) / s;
+  if (nprobe > cap.pcap) {
+    if (!last_resort) return kOverflow;
+    res.status = MH_READ_CAPACITY;
+    write_query(c, q, res);
+    return kDone;
+  }
+
+  // Probe every s-th window; compact present hashes into the hit list.
+  uint32_t nhits = 0;
+  for (uint32_t base = 0; base < nprobe; base += BS) {
+    const uint32_t p = base + tid;
+    uint32_t present = 0, cnt = 0;
+    uint64_t lo = 0;
+    if (p < nprobe) {
+      const uint32_t h = hash28(ws.q + p * s, k);
+      if (h == ix.empty_hash) {
+        present = 1;
+      } else {
+        lo = ix.dir[h];
+        const uint64_t hi = ix.dir[h + 1];
+        present = hi > lo;
+        cnt = (uint32_t)(hi - lo);
+      }
+    }
+    uint32_t tot;
+    const uint32_t idx = block_excl_scan<BS>(present, ws.red, tot);
+    if (present) {
+      ws.hit_j[nhits + idx] = p * s;
+      ws.hit_lo[nhits + idx] = lo;
+      ws.hit_base[nhits + idx] = cnt;
+    }
+    nhits += tot;
+  }
+  __syncthreads();
+  uint32_t E = 0;
+  for (uint32_t base = 0; base < nhits; base += BS) {
+    const uint32_t i = base + tid;
+    const uint32_t cnt = i < nhits ? ws.hit_base[i] : 0;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<BS>(cnt, ws.red, tot);
+    if (i < nhits) ws.hit_base[i] = E + ex;
+    E += tot;
+  }
+  if (tid == 0) ws.hit_base[nhits] = E;
+  __syncthreads();
+  if (E > cap.ecap) {
+    if (!last_resort) return kOverflow;
+    res.status = MH_READ_CAPACITY;
+    write_query(c, q, res);
+    return kDone;
+  }
+  if (tid == 0) {
+    atomicAdd(&c.out.ctr->probes, (unsigned long long)nprobe);
+    atomicAdd(&c.out.ctr->hits, (unsigned long long)nhits);
+    atomicAdd(&c.out.ctr->entries, (unsigned long long)E);
+  }
+  if (nhits == 0 || E == 0) {  // Analyser.cpp:1989-1990, :2012-2013
+    write_query(c, q, res);
+    return kDone;
+  }
+
+  // Gather bucket entries: k1 = ref << 32 | hit index, v1 = ref position.
+  for (uint32_t e = tid; e < E; e += BS) {
+    uint32_t lo = 0, hi = nhits;  // largest i with hit_base[i] <= e
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (ws.hit_base[mid] <= e) lo = mid;
+      else hi = mid;
+    }
+    const uint2 en = ix.ent[ws.hit_lo[lo] + (e - ws.hit_base[lo])];
+    ws.k1[e] = ((uint64_t)en.x << 32) | lo;
+    ws.v1[e] = en.y;
+  }
+  __syncthreads();
+
+  uint32_t nP = 0, nranks = 0, nchains = 0, G = 0;
+  bool direct = false;
+  if (!(c.flags & MH_CLASSIFY_NO_DIRECT) && nhits == nprobe && E == nhits && nhits >= 2) {
+    // Direct hit: every probe found a single-entry bucket, all entries on one
+    // diagonal of one reference.  The filters then keep everything and the
+    // hits form one chain, so the sorts are skipped.
+    const uint64_t r_ref = ws.k1[0] >> 32;
+    const uint32_t diag = ws.v1[0] - ws.hit_j[0];
+    int ok = 1;
+    for (uint32_t e = tid; e < E; e += BS)
+      ok &= ((ws.k1[e] >> 32) == r_ref) && (ws.v1[e] - ws.hit_j[e] == diag);
+    direct = __syncthreads_and(ok) != 0;
+    if (direct) {
+      for (uint32_t e = tid; e < E; e += BS) {
+        ws.k2[e] = (uint64_t)ws.v1[e];  // rank 0
+        ws.v2[e] = ws.hit_j[e];
+      }
+      if (tid == 0) {
+        ws.rref[0] = (uint32_t)r_ref;
+        ws.t3[0] = E;                      // chain maximum of rank 0
+        ws.v1[0] = 0;                      // chains of rank 0: [0, 1)
+        ws.v1[1] = 1;
+        ((uint32_t *)ws.k1)[0] = 0;        // chain 0: entries [0, E-1]
+        ((uint32_t *)ws.k1)[1] = E - 1;
+        atomicAdd(&c.out.ctr->direct, 1ull);
+      }
+      nP = E;
+      nranks = 1;
+      nchains = 1;
+      G = E;
+      __syncthreads();
+    }
+  }
+
+
+  if (!direct) {
+    // Successive-occurrence filter.  Sort by (ref, hit) and measure runs of
+    // consecutive hit indices; repeats of one hit inside a bucket do not
+    // lengthen a run (Analyser.cpp:2057-2058).
+    const uint32_t n2 = next_pow2(E);
+    for (uint32_t e = E + tid; e < n2; e += BS) {
+      ws.k1[e] = ~0ull;
+      ws.v1[e] = ~0u;
+    }
+    bitonic_sort<BS>(ws.k1, ws.v1, n2);
+    // t1 flags: bit0 run start, bit1 new hit index, bit2 reference start
+    for (uint32_t t = tid; t < E; t += BS) {
+      const uint64_t key = ws.k1[t];
+      uint32_t f = 7;
+      if (t > 0) {
+        const uint64_t pk = ws.k1[t - 1];
+        const bool same = (pk >> 32) == (key >> 32);
+        const uint32_t hit = (uint32_t)key, phit = (uint32_t)pk;
+        f = ((!same || (hit != phit && hit != phit + 1)) ? 1u : 0u) |
+            ((!same || hit != phit) ? 2u : 0u) | (!same ? 4u : 0u);
+      }
+      ws.t1[t] = f;
+    }
+    __syncthreads();
+    uint32_t nruns = 0;
+    for (uint32_t base = 0; base < E; base += BS) {
+      const uint32_t t = base + tid;
+      const uint32_t st = t < E ? (ws.t1[t] & 1u) : 0u;
+      uint32_t tot;
+      const uint32_t ex = block_excl_scan<BS>(st, ws.red, tot);
+      if (t < E) ws.t2[t] = nruns + ex + st - 1;  // run id
+      nruns += tot;
+    }
+    __syncthreads();
+    for (uint32_t r = tid; r < nruns; r += BS) ws.t3[r] = 0;
+    __syncthreads();
+    for (uint32_t t = tid; t < E; t += BS)
+      if (ws.t1[t] & 2u) atomicAdd(&ws.t3[ws.t2[t]], 1u);
+    __syncthreads();
+    uint32_t mx = 0;
+    for (uint32_t t = tid; t < E; t += BS) {
+      const uint32_t L = ws.t3[ws.t2[t]];
+      ws.t2[t] = L;  // length of the element's run
+      mx = max(mx, L);
+    }
+    const uint32_t maxrun = block_max<BS>(mx, ws.red);
+    // highestSuccesive moves only on a continued run (Analyser.cpp:2067-2072),
+    // and the cut is max(1, highest - 1) (:2099).
+    const uint32_t cut = maxrun >= 2 ? maxrun - 1 : 1;
+    __syncthreads();
+    uint32_t nrefseg = 0;
+    for (uint32_t base = 0; base < E; base += BS) {
+      const uint32_t t = base + tid;
+      const uint32_t st = t < E ? ((ws.t1[t] >> 2) & 1u) : 0u;
+      uint32_t tot;
+      const uint32_t ex = block_excl_scan<BS>(st, ws.red, tot);
+      if (t < E) ws.t3[t] = nrefseg + ex + st - 1;  // reference segment id
+      nrefseg += tot;
+    }
+    __syncthreads();
+    for (uint32_t r = tid; r < nrefseg; r += BS) ws.rref[r] = 0;
+    __syncthreads();
+    for (uint32_t t = tid; t < E; t += BS) atomicMax(&ws.rref[ws.t3[t]], ws.t2[t]);
+    __syncthreads();
+    for (uint32_t t = tid; t < E; t += BS) ws.t2[t] = ws.rref[ws.t3[t]] >= cut ? 1u : 0u;
+    __syncthreads();
+    // Kept pairs, ranked by reference: k2 = rank << 32 | refPos, v2 = qPos.
+    uint32_t nkeep = 0, nrank = 0;
+    for (uint32_t base = 0; base < E; base += BS) {
+      const uint32_t t = base + tid;
+      uint32_t keep = 0, rs = 0;
+      if (t < E) {
+        keep = ws.t2[t];
+        rs = keep & ((ws.t1[t] >> 2) & 1u);
+      }
+      uint32_t tk, tr;
+      const uint32_t ek = block_excl_scan<BS>(keep, ws.red, tk);
+      const uint32_t er = block_excl_scan<BS>(rs, ws.red, tr);
+      if (keep) {
+        const uint64_t key = ws.k1[t];
+        const uint32_t rank = nrank + er + rs - 1;
+        ws.k2[nkeep + ek] = ((uint64_t)rank << 32) | ws.v1[t];
+        ws.v2[nkeep + ek] = ws.hit_j[(uint32_t)key];
+        if (rs) ws.rref[rank] = (uint32_t)(key >> 32);
+      }
+      nkeep += tk;
+      nrank += tr;
+    }
+    nP = nkeep;
+    nranks = nrank;
+    // Sort by (reference, refPos, qPos) (Analyser.cpp:1024-1059).
+    const uint32_t p2 = next_pow2(nP);
+    __syncthreads();
+    for (uint32_t e = nP + tid; e < p2; e += BS) {
+      ws.k2[e] = ~0ull;
+      ws.v2[e] = ~0u;
+    }
+    bitonic_sort<BS>(ws.k2, ws.v2, p2);
+    // Chains: neighbours link when both coordinates step by s
+    // (Analyser.cpp:2351-2353, u32 arithmetic).
+    for (uint32_t t = tid; t < nP; t += BS) {
+      uint32_t st = 1;
+      if (t > 0) {
+        const uint64_t a = ws.k2[t - 1], b = ws.k2[t];
+        st = ((a >> 32) == (b >> 32) && (uint32_t)a == (uint32_t)b - (uint32_t)s &&
+              ws.v2[t - 1] == ws.v2[t] - (uint32_t)s)
+                 ? 0u
+                 : 1u;
+      }
+      ws.t1[t] = st;
+    }
+    __syncthreads();
+    uint32_t nch = 0;
+    for (uint32_t base = 0; base < nP; base += BS) {
+      const uint32_t t = base + tid;
+      const uint32_t st = t < nP ? ws.t1[t] : 0u;
+      uint32_t tot;
+      const uint32_t ex = block_excl_scan<BS>(st, ws.red, tot);
+      if (t < nP) ws.t2[t] = nch + ex + st - 1;  // chain id
+      nch += tot;
+    }
+    nchains = nch;
+    __syncthreads();
+    uint32_t *cfl = (uint32_t *)ws.k1;  // chain first/last element
+    for (uint32_t t = tid; t < nP; t += BS) {
+      const uint32_t cid = ws.t2[t];
+      if (ws.t1[t]) cfl[2 * cid] = t;
+      if (t + 1 == nP || ws.t1[t + 1]) cfl[2 * cid + 1] = t;
+      const uint32_t rank = (uint32_t)(ws.k2[t] >> 32);
+      if (t == 0 || (uint32_t)(ws.k2[t - 1] >> 32) != rank) ws.v1[rank] = cid;
+    }
+    for (uint32_t r = tid; r < nranks; r += BS) ws.t3[r] = 0;
+    __syncthreads();
+    for (uint32_t cid = tid; cid < nchains; cid += BS) {
+      const uint32_t f = cfl[2 * cid], l = cfl[2 * cid + 1];
+      atomicMax(&ws.t3[(uint32_t)(ws.k2[f] >> 32)], l - f + 1);
+    }
+    __syncthreads();
+    uint32_t gm = 0;
+    for (uint32_t r = tid; r < nranks; r += BS) gm = max(gm, ws.t3[r]);
+    G = block_max<BS>(gm, ws.red);
+    __syncthreads();
+  }
+
+  if (G < 2) {  // Analyser.cpp:2398-2403: no seeds, every candidate scores -2
+    write_query(c, q, res);
+    return kDone;
+  }
+
+  // Score every reference whose longest chain equals G (Analyser.cpp:2429-2436).
+  const bool ultrafast = (c.flags & MH_CLASSIFY_ULTRAFAST) != 0;
+  const uint32_t *cfl = (const uint32_t *)ws.k1;
+  float *rel_of = (float *)ws.t2;
+  uint32_t *valid_of = ws.t1;
+  uint32_t seeds_total = 0, seeded = 0;
+  for (uint32_t r = 0; r < nranks; r++) {
+    if (ws.t3[r] != G) {
+      if (tid == 0) valid_of[r] = 0;
+      continue;
+    }
+    const uint32_t c0 = ws.v1[r];
+    const uint32_t c1 = (r + 1 < nranks) ? ws.v1[r + 1] : nchains;
+    const uint32_t R = ws.rref[r];
+    const uint64_t ro = ix.ref_off[R];
+    const uint64_t ref_len = ix.ref_off[R + 1] - ro;
+    const uint8_t *rb = ix.bytes + ro;
+    auto seed = [&](uint32_t i, uint32_t &sref, uint32_t &sq, uint32_t &lref, uint32_t &lq) {
+      const uint32_t cid = c0 + i;
+      const uint32_t f = cfl[2 * cid], l = cfl[2 * cid + 1];
+      sref = (uint32_t)ws.k2[f];
+      sq = ws.v2[f];
+      lref = (uint32_t)ws.k2[l] + (uint32_t)s;  // Analyser.cpp:2465-2467
+      lq = ws.v2[l] + (uint32_t)s;
+    };
+    uint64_t sum_r, sum_c;
+    uint32_t hits;
+    score_ref<BS>(ws.q, m, rb, ref_len, c1 - c0, T, ultrafast, ws, seed, sum_r, sum_c, hits);
+    const float score = matrix_score(sum_r, sum_c, hits, m);
+    float rel = -1.0f;
+    uint32_t valid = 0, lut_bad = 0;
+    if (score > -1.0f) {  // MatrixCPU.cpp:356
+      if (rel_score(c.lut, m, ref_len, score, rel)) valid = 1;
+      else lut_bad = 1;
+    }
+    seeds_total += c1 - c0;
+    seeded++;
+    if (tid == 0) {
+      rel_of[r] = rel;
+      valid_of[r] = valid;
+      if (lut_bad) res.status |= MH_READ_CAPACITY;
+      if (c.out.cands) {
+        const unsigned long long slot = atomicAdd(&c.out.ctr->cand_used, 1ull);
+        if (slot < c.out.cand_cap) {
+          mh_cand cd;
+          cd.read = read;
+          cd.strand = strand;
+          cd.ref = R;
+          cd.hits = hits;
+          cd.sum_r = sum_r;
+          cd.sum_c = sum_c;
+          cd.score = score;
+          cd.rel = rel;
+          cd.n_seeds = c1 - c0;
+          cd.pad = 0;
+          c.out.cands[slot] = cd;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    atomicAdd(&c.out.ctr->seeds, (unsigned long long)seeds_total);
+    atomicAdd(&c.out.ctr->cands, (unsigned long long)seeded);
+  }
+
+  // Best hit: first strict maximum of the relative scores (all >= 0), and
+  // the references tied with it, in ascending reference order.
+  uint32_t nv = 0, hb = 0;
+  for (uint32_t r = tid; r < nranks; r += BS)
+    if (valid_of[r]) {
+      nv++;
+      hb = max(hb, __float_as_uint(rel_of[r]));
+    }
+  const uint32_t n_cand = block_sum<BS>(nv, ws.red);
+  const uint32_t best = block_max<BS>(hb, ws.red);
+  res.n_cand = n_cand;
+  if (n_cand > 0) {
+    res.highest = __uint_as_float(best);
+    uint32_t ntop = 0;
+    for (uint32_t base = 0; base < nranks; base += BS) {
+      const uint32_t r = base + tid;
+      const uint32_t tie = (r < nranks && valid_of[r] && __float_as_uint(rel_of[r]) == best);
+      uint32_t tot;
+      const uint32_t ex = block_excl_scan<BS>(tie, ws.red, tot);
+      if (tie) ws.t3[ntop + ex] = ws.rref[r];  // t3 is free once seeding is done
+      ntop += tot;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const unsigned long long off = atomicAdd(&c.out.ctr->top_used, (unsigned long long)ntop);
+      ws.red[40] = (uint32_t)off;
+      ws.red[41] = (off + ntop <= c.out.top_cap) ? 1u : 0u;
+    }
+    __syncthreads();
+    const uint32_t off = ws.red[40];
+    if (ws.red[41]) {
+      for (uint32_t i = tid; i < ntop; i += BS) c.out.top[off + i] = ws.t3[i];
+      res.n_top = ntop;
+      res.top_off = off;
+    } else {
+      res.n_top = ntop;
+      res.top_off = 0xFFFFFFFFu;  // pool exhausted; the host reports the size needed
+    }
+  }
+  write_query(c, q, res);
+  __syncthreads();
+  return kDone;
+}
+
+// One query per workgroup (plain form) or a persistent loop over a queue.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_classify(Ctx c, Caps cap, const uint32_t *qlist,
+                                                 const unsigned long long *qstart, uint32_t nq,
+                                                 const unsigned long long *nq_dev,
+                                                 unsigned long long *head) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const Ws ws = ws_carve(smem, cap);
+  if (qlist && qstart) qlist += *qstart;
+  if (head == nullptr) {
+    const uint32_t i = blockIdx.x;
+    if (i >= nq) return;
+    const uint32_t q = qlist ? qlist[i] : i;
+    if (process_query<BS>(c, ws, cap, q, false) == kOverflow && threadIdx.x == 0) {
+      const unsigned long long slot = atomicAdd(&c.out.ctr->overflow_n, 1ull);
+      if (slot < c.out.overflow_cap) c.out.overflow[slot] = q;
+    }
+    return;
+  }
+  const unsigned long long n = nq_dev ? *nq_dev : nq;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const unsigned long long i = atomicAdd(head, 1ull);
+      ws.red[48] = i < n ? (uint32_t)i : 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    const uint32_t i = ws.red[48];
+    if (i == 0xFFFFFFFFu) break;
+    const uint32_t q = qlist ? qlist[i] : i;
+    if (process_query<BS>(c, ws, cap, q, false) == kOverflow && threadIdx.x == 0) {
+      const unsigned long long slot = atomicAdd(&c.out.ctr->overflow_n, 1ull);
+      if (slot < c.out.overflow_cap) c.out.overflow[slot] = q;
+    }
+  }
+}
+
+// Queries that overflowed the LDS kernels, with the workspace in HBM.
+__global__ __launch_bounds__(1024) void k_classify_big(Ctx c, Caps cap, unsigned char *scratch,
+                                                       size_t ws_stride) {
+  const Ws ws = ws_carve(scratch + blockIdx.x * ws_stride, cap);
+  __shared__ uint32_t slot;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const unsigned long long n = c.out.ctr->overflow_n;
+      const unsigned long long i = atomicAdd(&c.out.ctr->big_overflow_n, 1ull);
+      slot = (i < n && i < c.out.overflow_cap) ? (uint32_t)i : 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    if (slot == 0xFFFFFFFFu) break;
+    process_query<1024>(c, ws, cap, c.out.overflow[slot], true);
+  }
+}
+
+// Strand merge (Analyser.cpp:592-611): the reverse strand wins only when its
+// HighestScore is strictly higher; a read is reported only with candidates
+// and a non-zero HighestScore (Analyser.cpp:388).
+__global__ void k_merge(const QueryOut *qo, uint32_t n_reads, mh_read_result *out) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n_reads;
+       r += gridDim.x * blockDim.x) {
+    const QueryOut f = qo[2 * r], b = qo[2 * r + 1];
+    const bool rc = f.highest < b.highest;
+    const QueryOut &w = rc ? b : f;
+    mh_read_result o;
+    o.highest = w.highest;
+    o.strand = rc ? 1u : 0u;
+    o.n_cand = w.n_cand;
+    const bool report = w.n_cand > 0 && w.highest != 0.0f;
+    o.n_top = report ? w.n_top : 0u;
+    o.top_off = report ? w.top_off : 0u;
+    o.status = f.status | b.status;
+    out[r] = o;
+  }
+}
+
+__global__ void k_zero(unsigned long long *p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+// Length buckets for the batched form: bucket of each query by read length.
+__global__ void k_bin_count(Reads rd, const uint32_t *max_len, uint32_t nb,
+                            unsigned long long *cnt) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < rd.n; r += gridDim.x * blockDim.x) {
+    const uint64_t m = rd.off[r + 1] - rd.off[r];
+    uint32_t b = 0;
+    while (b + 1 < nb && m > max_len[b]) b++;
+    atomicAdd(&cnt[b], 2ull);
+  }
+}
+
+__global__ void k_bin_scatter(Reads rd, const uint32_t *max_len, uint32_t nb,
+                              const unsigned long long *cnt, unsigned long long *cursor,
+                              uint32_t *list) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < rd.n; r += gridDim.x * blockDim.x) {
+    const uint64_t m = rd.off[r + 1] - rd.off[r];
+    uint32_t b = 0;
+    while (b + 1 < nb && m > max_len[b]) b++;
+    unsigned long long base = 0;
+    for (uint32_t i = 0; i < b; i++) base += cnt[i];
+    const unsigned long long at = atomicAdd(&cursor[b], 2ull);
+    list[base + at] = 2 * r;
+    list[base + at + 1] = 2 * r + 1;
+  }
+}
+
+// Per-bucket queue start offsets into the binned list.
+__global__ void k_bin_offsets(const unsigned long long *cnt, uint32_t nb, unsigned long long *start) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    unsigned long long a = 0;
+    for (uint32_t i = 0; i < nb; i++) {
+      start[i] = a;
+      a += cnt[i];
+    }
+  }
+}
+
+// Seed scoring only (drop-in for the scoring role of CUDAGetHighestScore).
+__global__ __launch_bounds__(64) void k_score_seeds(IndexView ix, Luts lut, const uint8_t *qb,
+                                                   const uint64_t *qoff, const uint32_t *pq,
+                                                   const uint32_t *pr, const uint64_t *soff,
+                                                   const uint32_t *seeds, uint32_t n_pairs,
+                                                   uint32_t qcap, uint32_t flags, float *o_score,
+                                                   float *o_rel, uint64_t *o_sr, uint64_t *o_sc,
+                                                   uint32_t *o_hits) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  Ws ws;
+  ws.q = smem;
+  ws.bitmap = (uint32_t *)(smem + al8(qcap));
+  ws.red = ws.bitmap + (qcap / 32 + 1);
+  ws.acc = (unsigned long long *)(ws.red + 64);
+  for (uint32_t p = blockIdx.x; p < n_pairs; p += gridDim.x) {
+    const uint32_t qi = pq[p], R = pr[p];
+    const uint64_t q0 = qoff[qi];
+    const uint64_t m64 = qoff[qi + 1] - q0;
+    float score = -2.0f, rel = -1.0f;
+    uint64_t sum_r = 0, sum_c = 0;
+    uint32_t hits = 0;
+    if (m64 <= qcap && R < ix.n_refs) {
+      const uint32_t m = (uint32_t)m64;
+      __syncthreads();
+      for (uint32_t x = threadIdx.x; x < m; x += 64) ws.q[x] = qb[q0 + x];
+      const uint64_t s0 = soff[p], s1 = soff[p + 1];
+      const uint64_t ro = ix.ref_off[R];
+      const uint64_t ref_len = ix.ref_off[R + 1] - ro;
+      auto seed = [&](uint32_t i, uint32_t &sref, uint32_t &sq, uint32_t &lref, uint32_t &lq) {
+        const uint32_t *sd = seeds + 4 * (s0 + i);
+        sref = sd[0];
+        sq = sd[1];
+        lref = sd[2];
+        lq = sd[3];
+      };
+      if (s1 > s0) {
+        score_ref<64>(ws.q, m, ix.bytes + ro, ref_len, (uint32_t)(s1 - s0), ix.T,
+                      (flags & MH_CLASSIFY_ULTRAFAST) != 0, ws, seed, sum_r, sum_c, hits);
+        score = matrix_score(sum_r, sum_c, hits, m);
+        if (score > -1.0f && !rel_score(lut, m, ref_len, score, rel)) rel = -1.0f;
+      }
+    }
+    if (threadIdx.x == 0) {
+      o_score[p] = score;
+      if (o_rel) o_rel[p] = rel;
+      if (o_sr) o_sr[p] = sum_r;
+      if (o_sc) o_sc[p] = sum_c;
+      if (o_hits) o_hits[p] = hits;
+    }
+  }
+}
+
+bool g_comp_ready[64] = {};
+
+hipError_t ensure_comp_table(int device) {
+  if (device >= 0 && device < 64 && g_comp_ready[device]) return hipSuccess;
+  uint8_t t[256] = {};
+  const char *from = "ATUGCYRSWKMBDHVN-~";
+  const char *to = "TAACGRYSWMKVHDBN-~";
+  for (int i = 0; from[i]; i++) t[(uint8_t)from[i]] = (uint8_t)to[i];
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(kComp), t, sizeof(t));
+  if (e == hipSuccess && device >= 0 && device < 64) g_comp_ready[device] = true;
+  return e;
+}
+
+template <int BS>
+hipError_t launch_lds(const Ctx &c, const Caps &cap, const uint32_t *qlist,
+                      const unsigned long long *qstart, uint32_t nq,
+                      const unsigned long long *nq_dev, unsigned long long *head, uint32_t grid,
+                      hipStream_t st) {
+  const size_t lds = ws_bytes(cap);
+  hipFuncSetAttribute((const void *)k_classify<BS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                      (int)lds);
+  hipLaunchKernelGGL((k_classify<BS>), dim3(grid), dim3(BS), lds, st, c, cap, qlist, qstart, nq,
+                     nq_dev, head);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile(const mh_tile &t, const Ctx &c, const Caps &cap, const uint32_t *qlist,
+                       const unsigned long long *qstart, uint32_t nq,
+                       const unsigned long long *nq_dev, unsigned long long *head, uint32_t grid,
+                       hipStream_t st) {
+  switch (t.block) {
+    case 64: return launch_lds<64>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
+    case 128: return launch_lds<128>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
+    case 256: return launch_lds<256>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
+    case 512: return launch_lds<512>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
+    default: return launch_lds<1024>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
+  }
+}
+
+__global__ void k_max_len(Reads rd, unsigned long long *out) {
+  uint32_t mx = 0;
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < rd.n; r += gridDim.x * blockDim.x)
+    mx = max(mx, (uint32_t)min(rd.off[r + 1] - rd.off[r], (uint64_t)0xFFFFFFFFu));
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)mx);
+}
+
+// LDS capacities of a tile, shrunk until the workspace fits `limit` bytes:
+// first the entry capacity (down to 256), then the query length.  Queries
+// beyond the capacities go to the large workspace.
+Caps caps_for(const mh_tile &t, int s, size_t limit) {
+  Caps c;
+  const uint32_t sd = (uint32_t)(s > 0 ? s : 1);
+  c.qcap = (t.max_len + 63) & ~63u;
+  c.pcap = c.qcap / sd + 2;
+  c.ecap = t.entries_cap;
+  while (ws_bytes(c) > limit && c.ecap > 256) c.ecap >>= 1;
+  while (ws_bytes(c) > limit && c.qcap > 64) {
+    c.qcap = ((c.qcap / 2) + 63) & ~63u;
+    c.pcap = c.qcap / sd + 2;
+  }
+  return c;
+}
+
+constexpr uint32_t kBigBlocks = 32;
+constexpr uint32_t kBigEcap = 1u << 19;
+
+}  // namespace
+
+std::vector<mh_tile> default_tiles(uint32_t max_read_len) {
+  std::vector<mh_tile> t;
+  t.push_back(mh_tile{320, 64, 256, 0});
+  t.push_back(mh_tile{1024, 128, 512, 0});
+  t.push_back(mh_tile{4096, 256, 2048, 0});
+  t.push_back(mh_tile{std::max<uint32_t>(max_read_len, 4097), 256, 2048, 0});
+  return t;
+}
+
+// Scratch layout for one classify call (bytes from the handle's arena):
+//   QueryOut[2n] | overflow list u32[2n] | binned list u32[2n] |
+//   bucket words (u64[64]) | large-kernel workspaces
+size_t classify_scratch_bytes(uint32_t n_reads, uint32_t max_read_len, int s) {
+  const size_t nq = 2ull * n_reads;
+  Caps big;
+  big.qcap = (max_read_len + 63) & ~63u;
+  big.pcap = big.qcap / (uint32_t)(s > 0 ? s : 1) + 2;
+  big.ecap = kBigEcap;
+  size_t b = al8(nq * sizeof(QueryOut)) + 2 * al8(nq * 4) + 64 * 8;
+  b = (b + 255) & ~size_t(255);
+  b += kBigBlocks * ((ws_bytes(big) + 255) & ~size_t(255));
+  return b;
+}
+
+mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool batched) {
+  hipError_t e = ensure_comp_table(h->device);
+  if (e != hipSuccess) return hip_error(h, e, "complement table");
+  const uint32_t n = a.rd.n;
+  const uint64_t nq = 2ull * n;
+  const size_t need = classify_scratch_bytes(n, h->max_read_len, a.ix.s);
+  mh_status st = ensure_scratch(h, x, need);
+  if (st) return st;
+  unsigned char *base = (unsigned char *)x.d_scratch;
+  size_t o = 0;
+  QueryOut *qo = (QueryOut *)(base + o);
+  o += al8(nq * sizeof(QueryOut));
+  uint32_t *overflow = (uint32_t *)(base + o);
+  o += al8(nq * 4);
+  uint32_t *binned = (uint32_t *)(base + o);
+  o += al8(nq * 4);
+  unsigned long long *bw = (unsigned long long *)(base + o);  // [0..7] cnt, [8..15] cursor, [16..23] start
+  o += 64 * 8;
+  o = (o + 255) & ~size_t(255);
+  unsigned char *big_ws = base + o;
+
+  Ctx c;
+  c.ix = a.ix;
+  c.lut = a.luts;
+  c.rd = a.rd;
+  c.flags = a.flags;
+  c.out = a.out;
+  c.out.q = qo;
+  c.out.ctr = x.d_ctr;
+  c.out.overflow = overflow;
+  c.out.overflow_cap = (uint32_t)nq;
+  hipStream_t s = x.stream;
+  const uint32_t zn = sizeof(DevCounters) / 8;
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, (unsigned long long *)x.d_ctr, zn);
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, bw, 64u);
+  if (nq) {
+    std::vector<mh_tile> tiles = h->tiles.empty() ? default_tiles(h->max_read_len) : h->tiles;
+    if (!batched) {
+      // Plain form: one launch per batch, one workgroup per query, with the
+      // single tile that covers the longest read of the batch.
+      hipLaunchKernelGGL(k_max_len, dim3(std::min<uint32_t>((n + 255) / 256, 1024)), dim3(256),
+                         0, s, a.rd, bw + 24);
+      unsigned long long mlen = 0;
+      e = hipMemcpyAsync(&mlen, bw + 24, 8, hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) return hip_error(h, e, "max read length");
+      size_t ti = 0;
+      while (ti + 1 < tiles.size() && mlen > tiles[ti].max_len) ti++;
+      const mh_tile &t0 = tiles[ti];
+      const Caps cap = caps_for(t0, a.ix.s, h->lds_per_block);
+      e = launch_tile(t0, c, cap, nullptr, nullptr, (uint32_t)nq, nullptr, nullptr, (uint32_t)nq,
+                      s);
+      if (e != hipSuccess) return hip_error(h, e, "k_classify");
+    } else {
+      const uint32_t nb = (uint32_t)std::min<size_t>(tiles.size(), 8);
+      uint32_t *d_maxlen = (uint32_t *)(bw + 32);  // 8 words inside the bucket area
+      uint32_t ml[8];
+      for (uint32_t i = 0; i < nb; i++) ml[i] = tiles[i].max_len;
+      e = hipMemcpyAsync(d_maxlen, ml, nb * 4, hipMemcpyHostToDevice, s);
+      if (e != hipSuccess) return hip_error(h, e, "tile lengths");
+      const Reads rd = a.rd;
+      hipLaunchKernelGGL(k_bin_count, dim3(std::min<uint32_t>((n + 255) / 256, 4096)), dim3(256),
+                         0, s, rd, d_maxlen, nb, bw);
+      hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, bw, nb, bw + 16);
+      hipLaunchKernelGGL(k_bin_scatter, dim3(std::min<uint32_t>((n + 255) / 256, 4096)),
+                         dim3(256), 0, s, rd, d_maxlen, nb, bw, bw + 8, binned);
+      // Long buckets first so that their tail overlaps with the short ones.
+      for (int b = (int)nb - 1; b >= 0; b--) {
+        const mh_tile &t = tiles[b];
+        const Caps cap = caps_for(t, a.ix.s, h->lds_per_block);
+        uint32_t grid = t.grid;
+        if (!grid) {
+          const size_t lds = ws_bytes(cap);
+          const uint32_t per_cu = (uint32_t)std::max<size_t>(
+              1, std::min<size_t>(h->lds_per_block / std::max<size_t>(lds, 1), 2048 / t.block));
+          grid = h->cu_count * per_cu;
+        }
+        // queue b: binned[start[b], start[b] + cnt[b]), head queue_head[b]
+        e = launch_tile(t, c, cap, binned, bw + 16 + b, 0, bw + b, &x.d_ctr->queue_head[b],
+                        grid, s);
+        if (e != hipSuccess) return hip_error(h, e, "k_classify batched");
+      }
+    }
+    Caps big;
+    big.qcap = (h->max_read_len + 63) & ~63u;
+    big.pcap = big.qcap / (uint32_t)(a.ix.s > 0 ? a.ix.s : 1) + 2;
+    big.ecap = kBigEcap;
+    const size_t stride = (ws_bytes(big) + 255) & ~size_t(255);
+    hipLaunchKernelGGL(k_classify_big, dim3(kBigBlocks), dim3(1024), 0, s, c, big, big_ws, stride);
+    hipLaunchKernelGGL(k_merge, dim3(std::min<uint32_t>((n + 255) / 256, 4096)), dim3(256), 0, s,
+                       (const QueryOut *)qo, n, a.final_out);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(h, e, "classify launch");
+  }
+  return MH_OK;
+}
+
+void counters_from(const DevCounters &d, uint64_t n_reads, mh_counters &o) {
+  o.queries = 2 * n_reads;
+  o.probes = d.probes;
+  o.hits = d.hits;
+  o.entries = d.entries;
+  o.seeds = d.seeds;
+  o.cands = d.cands;
+  o.direct = d.direct;
+  o.overflow = d.overflow_n;
+}
+
+mh_status launch_score_seeds(mh_handle *h, const IndexView &ix, const Luts &luts,
+                             const uint8_t *d_q, const uint64_t *d_qoff, const uint32_t *d_pq,
+                             const uint32_t *d_pr, const uint64_t *d_soff, const uint32_t *d_seeds,
+                             uint32_t n_pairs, uint32_t flags, float *d_score, float *d_rel,
+                             uint64_t *d_sum_r, uint64_t *d_sum_c, uint32_t *d_hits) {
+  if (!n_pairs) return MH_OK;
+  const uint32_t qcap = (h->max_read_len + 63) & ~63u;
+  const size_t lds = al8(qcap) + (qcap / 32 + 1) * 4 + 64 * 4 + 16;
+  const uint32_t grid = std::min<uint32_t>(n_pairs, 65536);
+  hipLaunchKernelGGL(k_score_seeds, dim3(grid), dim3(64), lds, h->stream, ix, luts, d_q, d_qoff,
+                     d_pq, d_pr, d_soff, d_seeds, n_pairs, qcap, flags, d_score, d_rel, d_sum_r,
+                     d_sum_c, d_hits);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_error(h, e, "k_score_seeds");
+  return MH_OK;
+}
+
+}  // namespace mh

@@ -1,0 +1,190 @@
+// Internal types shared by the HIP translation units of libmirage_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mirage_hip.h"
+
+namespace mh {
+
+constexpr uint32_t kHashBits = 28;                    // MirAge/Hashmap.h:111
+constexpr uint32_t kHashMask = (1u << kHashBits) - 1u;
+constexpr uint64_t kDirSize = 1ull << kHashBits;      // direct-indexed buckets
+constexpr uint32_t kNoHash = 0xFFFFFFFFu;
+
+// Device view of an index.  Buckets are direct-indexed by the 28-bit hash:
+// entries of hash h are ent[dir[h], dir[h+1]) as (ref, pos) pairs.
+struct IndexView {
+  const uint64_t *dir;
+  const uint2 *ent;
+  const uint8_t *bytes;     // uppercased reference bases
+  const uint64_t *ref_off;  // n_refs + 1
+  uint32_t n_refs;
+  uint32_t empty_hash;      // present bucket with no entries, or kNoHash
+  int32_t T, s, k;
+};
+
+// Tables of glibc logf for the relative score (MatrixResult.cpp:21-29):
+// x_log[i] = logf(asfloat(0x3f800000 + i)), smax_log[L] = logf(scoreMax(L)).
+struct Luts {
+  const float *x_log;
+  const float *smax_log;
+  uint32_t n_x;
+  uint32_t n_smax;
+};
+
+struct Reads {
+  const uint8_t *bytes;
+  const uint64_t *off;
+  uint32_t n;
+};
+
+// Per read strand (query q = 2 * read + strand).
+struct QueryOut {
+  float highest;
+  uint32_t n_top;
+  uint32_t top_off;
+  uint32_t n_cand;
+  uint32_t status;
+  uint32_t pad[3];
+};
+
+// Device counters, zeroed before each classify call.
+struct DevCounters {
+  unsigned long long top_used;
+  unsigned long long cand_used;
+  unsigned long long overflow_n;
+  unsigned long long big_overflow_n;
+  unsigned long long probes, hits, entries, seeds, cands, direct;
+  unsigned long long queue_head[16];
+};
+
+struct Outputs {
+  QueryOut *q;
+  uint32_t *top;
+  uint64_t top_cap;
+  mh_cand *cands;
+  uint64_t cand_cap;
+  DevCounters *ctr;
+  uint32_t *overflow;      // queries deferred to the large workspace
+  uint32_t overflow_cap;
+};
+
+struct Caps {
+  uint32_t qcap;   // query bytes
+  uint32_t pcap;   // probe hits
+  uint32_t ecap;   // bucket entries (power of two)
+};
+
+}  // namespace mh
+
+struct mh_index {
+  mh_handle *owner = nullptr;
+  int device = 0;
+  int32_t threshold = 0;
+  uint32_t n_refs = 0;
+  uint32_t empty_hash = mh::kNoHash;
+  uint64_t n_entries = 0;
+  uint64_t n_buckets = 0;
+  uint64_t total_bytes = 0;
+  uint64_t *d_dir = nullptr;
+  uint2 *d_ent = nullptr;
+  uint8_t *d_bytes = nullptr;
+  uint64_t *d_ref_off = nullptr;
+  std::vector<uint64_t> h_ref_off;
+  mh::IndexView view() const {
+    mh::IndexView v;
+    v.dir = d_dir;
+    v.ent = d_ent;
+    v.bytes = d_bytes;
+    v.ref_off = d_ref_off;
+    v.n_refs = n_refs;
+    v.empty_hash = empty_hash;
+    v.T = threshold;
+    v.s = threshold / 2;
+    v.k = threshold - threshold / 2 + 1;
+    return v;
+  }
+};
+
+namespace mh {
+// One execution context: a stream with its own scratch arena and counters.
+// A handle has one; the serving loop keeps one per batch in flight.
+struct Exec {
+  hipStream_t stream = nullptr;
+  void *d_scratch = nullptr;
+  size_t scratch_bytes = 0;
+  DevCounters *d_ctr = nullptr;
+  DevCounters *h_ctr = nullptr;  // pinned host copy
+};
+}  // namespace mh
+
+struct mh_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;  // == ex.stream
+  uint32_t max_read_len = 32768;
+  uint32_t cu_count = 256;
+  size_t lds_per_block = 65536;
+  std::string last_error;
+  std::mutex mu;
+  // logf tables
+  float *d_x_log = nullptr;
+  float *d_smax_log = nullptr;
+  uint32_t n_x = 0, n_smax = 0;
+  mh::Exec ex;
+  mh_counters last{};
+  bool last_valid = false;
+  std::vector<mh_tile> tiles;
+  mh::Luts luts() const { return mh::Luts{d_x_log, d_smax_log, n_x, n_smax}; }
+};
+
+namespace mh {
+
+// Error helpers (host).
+mh_status set_error(mh_handle *h, mh_status st, const std::string &msg);
+mh_status hip_error(mh_handle *h, hipError_t e, const char *what);
+#define MH_HIP(h, call)                                      \
+  do {                                                       \
+    hipError_t _e = (call);                                  \
+    if (_e != hipSuccess) return ::mh::hip_error((h), _e, #call); \
+  } while (0)
+
+// Execution contexts (mh_api.hip).
+mh_status exec_init(mh_handle *h, Exec &x);
+void exec_free(Exec &x);
+// Grows x's scratch arena to at least `bytes` (synchronises x's stream first).
+mh_status ensure_scratch(mh_handle *h, Exec &x, size_t bytes);
+
+// Index build (mh_index.hip).
+mh_status build_index(mh_handle *h, const uint8_t *h_ref_bytes, const uint64_t *h_ref_off,
+                      uint32_t n_refs, int32_t threshold, uint32_t flags, mh_index **out);
+
+// Classification launchers (mh_classify.hip).
+struct ClassifyArgs {
+  IndexView ix;
+  Luts luts;
+  Reads rd;
+  uint32_t flags;
+  Outputs out;
+  mh_read_result *final_out;
+};
+mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool batched);
+// Scratch bytes one classify call needs.
+size_t classify_scratch_bytes(uint32_t n_reads, uint32_t max_read_len, int s);
+// Reads the counters of the last call on x (x's stream must be idle).
+void counters_from(const DevCounters &d, uint64_t n_reads, mh_counters &out);
+mh_status launch_score_seeds(mh_handle *h, const IndexView &ix, const Luts &luts,
+                             const uint8_t *d_q, const uint64_t *d_qoff, const uint32_t *d_pq,
+                             const uint32_t *d_pr, const uint64_t *d_soff, const uint32_t *d_seeds,
+                             uint32_t n_pairs, uint32_t flags, float *d_score, float *d_rel,
+                             uint64_t *d_sum_r, uint64_t *d_sum_c, uint32_t *d_hits);
+
+// Default tiles (mh_classify.hip).
+std::vector<mh_tile> default_tiles(uint32_t max_read_len);
+
+}  // namespace mh

@@ -1,0 +1,277 @@
+"""ctypes binding of libmirage_hip (include/mirage_hip.h).
+
+Device memory for inputs and outputs is held in torch tensors (plumbing only);
+every computation runs in the HIP library.  Loading fails loudly when the
+library has not been built: there is no fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _build
+
+MH_OK = 0
+MH_ERR_OVERFLOW = -5
+MH_INDEX_LAST_BUCKET_EMPTY = 1
+MH_CLASSIFY_ULTRAFAST = 1
+MH_CLASSIFY_NO_DIRECT = 2
+MH_CLASSIFY_BATCHED = 4
+
+READ_RESULT_DTYPE = np.dtype([("highest", "<f4"), ("strand", "<u4"), ("n_top", "<u4"),
+                              ("top_off", "<u4"), ("n_cand", "<u4"), ("status", "<u4")])
+CAND_DTYPE = np.dtype([("read", "<u4"), ("strand", "<u4"), ("ref", "<u4"), ("hits", "<u4"),
+                       ("sum_r", "<u8"), ("sum_c", "<u8"), ("score", "<f4"), ("rel", "<f4"),
+                       ("n_seeds", "<u4"), ("pad", "<u4")])
+COUNTER_FIELDS = ("queries", "probes", "hits", "entries", "seeds", "cands", "direct", "overflow")
+
+
+class IndexInfo(ctypes.Structure):
+    _fields_ = [("n_entries", ctypes.c_uint64), ("n_buckets", ctypes.c_uint64),
+                ("device_bytes", ctypes.c_uint64), ("n_refs", ctypes.c_uint32),
+                ("threshold", ctypes.c_int32), ("empty_hash", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32)]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_uint64) for f in COUNTER_FIELDS]
+
+
+class Tile(ctypes.Structure):
+    _fields_ = [("max_len", ctypes.c_uint32), ("block", ctypes.c_uint32),
+                ("entries_cap", ctypes.c_uint32), ("grid", ctypes.c_uint32)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_uint64), ("max_inflight", ctypes.c_uint32),
+                ("max_batch_reads", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("request_id", ctypes.c_uint64), ("n_reads", ctypes.c_uint32),
+                ("status", ctypes.c_int32), ("submit_s", ctypes.c_double),
+                ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double)]
+
+
+# Every symbol declared in include/mirage_hip.h.
+EXPORTS = ("mh_open", "mh_close", "mh_last_error", "mh_stream", "mh_max_read_len",
+           "mh_max_batch_reads", "mh_index_build", "mh_index_free", "mh_index_get_info",
+           "mh_index_bucket", "mh_classify", "mh_classify_host", "mh_get_counters",
+           "mh_score_seeds", "mh_set_tiles", "mh_get_tiles", "mh_autotune", "mh_server_start",
+           "mh_server_submit", "mh_server_wait", "mh_server_stop")
+
+_lib = None
+
+
+class MirageError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{msg} (status {status})")
+        self.status = status
+
+
+def load():
+    """Loads libmirage_hip.so; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = _build.hip_lib_path()
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run __graft_entry__.build() first")
+    L = ctypes.CDLL(path)
+    P, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32
+    sig = {
+        "mh_open": (i32, [i32, u32, ctypes.POINTER(P)]),
+        "mh_close": (i32, [P]),
+        "mh_last_error": (ctypes.c_char_p, [P]),
+        "mh_stream": (P, [P]),
+        "mh_max_read_len": (u32, [P]),
+        "mh_max_batch_reads": (u64, [P]),
+        "mh_index_build": (i32, [P, P, P, u32, i32, u32, ctypes.POINTER(P)]),
+        "mh_index_free": (i32, [P]),
+        "mh_index_get_info": (i32, [P, ctypes.POINTER(IndexInfo)]),
+        "mh_index_bucket": (i32, [P, P, u32, P, P, u64, ctypes.POINTER(ctypes.c_int64)]),
+        "mh_classify": (i32, [P, P, P, P, u32, u32, P, P, u64, P, u64, P, P]),
+        "mh_classify_host": (i32, [P, P, P, P, u32, u32, P, P, u64, P, u64, P, P]),
+        "mh_get_counters": (i32, [P, ctypes.POINTER(Counters)]),
+        "mh_score_seeds": (i32, [P, P, P, P, P, P, P, P, u32, u32, P, P, P, P, P]),
+        "mh_set_tiles": (i32, [P, P, u32]),
+        "mh_get_tiles": (i32, [P, P, u32, ctypes.POINTER(u32)]),
+        "mh_autotune": (i32, [P, P, P, P, u32, u32]),
+        "mh_server_start": (i32, [P, P, ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]),
+        "mh_server_submit": (i32, [P, P, P, u32, ctypes.POINTER(u64)]),
+        "mh_server_wait": (i32, [P, u64, ctypes.c_double, P, P, u64, ctypes.POINTER(u64),
+                                 ctypes.POINTER(RequestInfo)]),
+        "mh_server_stop": (i32, [P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name, None)
+        if f is None:
+            continue
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _dev_ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Handle:
+    """One device, one HIP stream (mh_open)."""
+
+    def __init__(self, device=0, max_read_len=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        st = self.lib.mh_open(device, max_read_len, ctypes.byref(h))
+        if st != MH_OK:
+            raise MirageError(st, "mh_open failed")
+        self.h = h
+        self.device = device
+
+    def check(self, st, what):
+        if st != MH_OK:
+            raise MirageError(st, f"{what}: {self.lib.mh_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            self.lib.mh_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def max_read_len(self):
+        return self.lib.mh_max_read_len(self.h)
+
+    def stream_ptr(self):
+        return self.lib.mh_stream(self.h)
+
+    def build_index(self, ref_bytes, ref_off, threshold, last_bucket_empty=True):
+        return DeviceIndex(self, ref_bytes, ref_off, threshold, last_bucket_empty)
+
+    def set_tiles(self, tiles):
+        arr = (Tile * len(tiles))(*[Tile(*t) for t in tiles])
+        self.check(self.lib.mh_set_tiles(self.h, ctypes.cast(arr, ctypes.c_void_p), len(tiles)),
+                   "mh_set_tiles")
+
+    def get_tiles(self):
+        arr = (Tile * 8)()
+        n = ctypes.c_uint32()
+        self.check(self.lib.mh_get_tiles(self.h, ctypes.cast(arr, ctypes.c_void_p), 8,
+                                         ctypes.byref(n)), "mh_get_tiles")
+        return [(arr[i].max_len, arr[i].block, arr[i].entries_cap, arr[i].grid)
+                for i in range(n.value)]
+
+    def counters(self):
+        c = Counters()
+        self.check(self.lib.mh_get_counters(self.h, ctypes.byref(c)), "mh_get_counters")
+        return {f: getattr(c, f) for f in COUNTER_FIELDS}
+
+
+class DeviceIndex:
+    """Bucketed k-mer index resident in HBM (mh_index_build)."""
+
+    def __init__(self, handle, ref_bytes, ref_off, threshold, last_bucket_empty=True):
+        self.handle = handle
+        rb = np.ascontiguousarray(ref_bytes, dtype=np.uint8)
+        ro = np.ascontiguousarray(ref_off, dtype=np.uint64)
+        ix = ctypes.c_void_p()
+        flags = MH_INDEX_LAST_BUCKET_EMPTY if last_bucket_empty else 0
+        handle.check(handle.lib.mh_index_build(handle.h, _np_ptr(rb), _np_ptr(ro), len(ro) - 1,
+                                               threshold, flags, ctypes.byref(ix)),
+                     "mh_index_build")
+        self.ix = ix
+        self.threshold = threshold
+        self.n_refs = len(ro) - 1
+
+    def free(self):
+        if self.ix:
+            self.handle.lib.mh_index_free(self.ix)
+            self.ix = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def info(self):
+        i = IndexInfo()
+        self.handle.check(self.handle.lib.mh_index_get_info(self.ix, ctypes.byref(i)),
+                          "mh_index_get_info")
+        return {f: getattr(i, f) for f, _ in IndexInfo._fields_ if f != "pad"}
+
+    def bucket(self, h):
+        n = ctypes.c_int64()
+        lib = self.handle.lib
+        self.handle.check(lib.mh_index_bucket(self.handle.h, self.ix, h, None, None, 0,
+                                              ctypes.byref(n)), "mh_index_bucket")
+        if n.value < 0:
+            return None
+        refs = np.zeros(max(n.value, 1), np.uint32)
+        pos = np.zeros(max(n.value, 1), np.uint32)
+        self.handle.check(lib.mh_index_bucket(self.handle.h, self.ix, h, _np_ptr(refs),
+                                              _np_ptr(pos), n.value, ctypes.byref(n)),
+                          "mh_index_bucket")
+        return list(zip(refs[:n.value].tolist(), pos[:n.value].tolist()))
+
+    def classify_host(self, reads, read_off, flags=0, top_cap=None, want_cands=False,
+                      cand_cap=None):
+        """Full path from host arrays (mh_classify_host).  Returns
+        (results, top_pool, cands)."""
+        lib, h = self.handle.lib, self.handle.h
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        read_off = np.ascontiguousarray(read_off, dtype=np.uint64)
+        n = len(read_off) - 1
+        top_cap = top_cap or max(64, 16 * n)
+        cand_cap = cand_cap or max(64, 16 * n)
+        for _ in range(3):
+            out = np.zeros(n, READ_RESULT_DTYPE)
+            top = np.zeros(top_cap, np.uint32)
+            cands = np.zeros(cand_cap, CAND_DTYPE) if want_cands else None
+            tn, cn = ctypes.c_uint64(), ctypes.c_uint64()
+            st = lib.mh_classify_host(h, self.ix, _np_ptr(reads), _np_ptr(read_off), n, flags,
+                                      _np_ptr(out), _np_ptr(top), top_cap, _np_ptr(cands),
+                                      cand_cap if want_cands else 0, ctypes.byref(tn),
+                                      ctypes.byref(cn))
+            if st == MH_ERR_OVERFLOW:
+                top_cap = max(top_cap, int(tn.value) + 16)
+                cand_cap = max(cand_cap, int(cn.value) + 16)
+                continue
+            self.handle.check(st, "mh_classify_host")
+            if want_cands:
+                cands = cands[:cn.value]
+            return out, top[:tn.value], cands
+        raise MirageError(MH_ERR_OVERFLOW, "output pools kept overflowing")
+
+    def classify_device(self, d_reads, d_read_off, n_reads, d_out, d_top, top_cap, flags=0,
+                        d_cands=None, cand_cap=0, wait=True):
+        """Full path on device-resident torch tensors (mh_classify)."""
+        lib, h = self.handle.lib, self.handle.h
+        tn, cn = ctypes.c_uint64(), ctypes.c_uint64()
+        st = lib.mh_classify(h, self.ix, _dev_ptr(d_reads), _dev_ptr(d_read_off), n_reads, flags,
+                             _dev_ptr(d_out), _dev_ptr(d_top), top_cap, _dev_ptr(d_cands),
+                             cand_cap, ctypes.byref(tn) if wait else None,
+                             ctypes.byref(cn) if wait else None)
+        self.handle.check(st, "mh_classify")
+        return (tn.value, cn.value) if wait else None
+
+
+def tie_sets(out, top):
+    """Per-read sorted top-tie tuples from a result array and its pool."""
+    res = []
+    for r in out:
+        n = int(r["n_top"])
+        o = int(r["top_off"])
+        res.append(tuple(sorted(top[o:o + n].tolist())) if n else ())
+    return res

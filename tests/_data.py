@@ -1,0 +1,37 @@
+"""Shared test data: golden fixtures and seeded workloads."""
+import glob
+import gzip
+import json
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def pack(seqs):
+    off = np.zeros(len(seqs) + 1, np.uint64)
+    off[1:] = np.cumsum([len(s) for s in seqs])
+    by = np.frombuffer("".join(seqs).encode(), np.uint8).copy() if seqs else np.zeros(0, np.uint8)
+    return by, off
+
+
+def golden_names():
+    return sorted(os.path.basename(p)[4:-8] for p in glob.glob(os.path.join(GOLDEN, "ref_*.json.gz")))
+
+
+def load_golden(name):
+    with gzip.open(os.path.join(GOLDEN, f"ref_{name}.json.gz"), "rt") as f:
+        fx = json.load(f)
+    rb, ro = pack(fx["refs"])
+    qb, qo = pack(fx["reads"])
+    return fx, rb, ro, qb, qo
+
+
+def expected_lines(fx, n_reads):
+    """Per read: None (no line printed) or (rel string, sorted ref ids)."""
+    out = []
+    for i in range(n_reads):
+        e = fx["expected"].get(str(i))
+        out.append(None if e is None else (e[0], tuple(e[1])))
+    return out

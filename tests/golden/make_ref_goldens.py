@@ -66,6 +66,19 @@ def datasets():
            D.RefSpec(n=16, lengths=L(mode=0, a=8000)),
            D.ReadSpec(n=300, lengths=L(mode=0, a=150), sub_rate=0.01, rc_frac=0.5),
            106, "n_runs")
+    yield ("t30_rc_copies", 30,
+           D.RefSpec(n=24, lengths=L(mode=0, a=6000)),
+           D.ReadSpec(n=400, lengths=L(mode=0, a=150), sub_rate=0.01, rc_frac=0.5),
+           108, "rc_copies")
+    yield ("t22_low_complexity", 22,
+           D.RefSpec(n=20, lengths=L(mode=0, a=6000)),
+           D.ReadSpec(n=400, lengths=L(mode=0, a=150), sub_rate=0.005, rc_frac=0.5),
+           109, "low_complexity")
+    yield ("t30_variant_copies", 30,
+           D.RefSpec(n=30, lengths=L(mode=0, a=5000)),
+           D.ReadSpec(n=400, lengths=L(mode=2, a=100, b=400), sub_rate=0.01, rc_frac=0.5),
+           110, "variant_copies")
+    yield ("t30_engineered", 30, None, None, 111, "engineered")
     yield ("t22_tiny_mixed", 22,
            D.RefSpec(n=8, lengths=L(mode=2, a=300, b=3000)),
            D.ReadSpec(n=250, lengths=L(mode=2, a=40, b=400), p_exact=0.2, p_random=0.1,
@@ -73,7 +86,49 @@ def datasets():
            107, None)
 
 
+COMP = np.zeros(256, np.uint8)
+for a, b in zip(b"ACGTN", b"TGCAN"):
+    COMP[a] = b
+
+
 def apply_edits(kind, rb, ro, qb, qo, rng):
+    """Edits applied to the references before the reads are drawn."""
+    nref = len(ro) - 1
+    if kind == "rc_copies":
+        # The reverse complement of a segment of ref 2i written into ref 2i+1:
+        # a read of the segment scores the same on both strands.
+        for r in range(0, nref - 1, 2):
+            a, b = int(ro[r]), int(ro[r + 2])
+            seg_len = int(rng.integers(800, 2000))
+            src = int(ro[r]) + int(rng.integers(0, int(ro[r + 1] - ro[r]) - seg_len))
+            dst = int(ro[r + 1]) + int(rng.integers(0, int(ro[r + 2] - ro[r + 1]) - seg_len))
+            rb[dst:dst + seg_len] = COMP[rb[src:src + seg_len][::-1]]
+            del a, b
+        return
+    if kind == "low_complexity":
+        # Homopolymers and short tandem repeats: the off-diagonal head
+        # comparison then runs to its boundary.
+        for r in range(nref):
+            a, b = int(ro[r]), int(ro[r + 1])
+            for _ in range(25):
+                at = a + int(rng.integers(0, b - a - 120))
+                ln = int(rng.integers(20, 110))
+                unit = rng.choice(list(b"ACGT"), int(rng.integers(1, 4))).astype(np.uint8)
+                rb[at:at + ln] = np.resize(unit, ln)
+        return
+    if kind == "variant_copies":
+        # Copies of a few segments with sparse substitutions, so that several
+        # references share runs and chains of nearly the same length.
+        segs = [rng.choice(list(b"ACGT"), int(rng.integers(600, 1500))).astype(np.uint8)
+                for _ in range(4)]
+        for r in range(nref):
+            a, b = int(ro[r]), int(ro[r + 1])
+            sg = segs[r % 4].copy()
+            flips = rng.random(len(sg)) < 0.01 * (r % 5)
+            sg[flips] = rng.choice(list(b"ACGT"), int(flips.sum()))
+            at = a + int(rng.integers(0, b - a - len(sg)))
+            rb[at:at + len(sg)] = sg
+        return
     if kind != "n_runs":
         return
     # N runs inside some references (window cutoff, Hashmap.cpp:2882-2895) and
@@ -86,6 +141,55 @@ def apply_edits(kind, rb, ro, qb, qo, rng):
         a = int(qo[i])
         rb_at = a + int(rng.integers(0, int(qo[i + 1]) - a))
         qb[rb_at] = ord("N")
+
+
+def engineered(seed, T=30):
+    """References built around chosen reads (not sampled), to reach the
+    candidate filters.  Per read R:
+      A: R with a substitution in its last probe window (run M-1, chain M-1),
+      B: every probe k-mer of R, shuffled, between random spacers (run M,
+         short chains),
+      C: R with a 3-base insertion in the middle (run M, two chains),
+      D: the first five probe windows of R, then the remaining probe k-mers
+         off the diagonal (run M, chain 5);
+    and per read R2:
+      P: R2 up to the end of probe window 4 plus a tail's worth (one chain of 5),
+      Q: R2 with one substitution inside window 4 (chains of 4 and 3)."""
+    rng = np.random.default_rng(seed)
+    s = T // 2
+    k = T - s + 1
+    rand = lambda n: rng.choice(list(b"ACGT"), n).astype(np.uint8)  # noqa: E731
+    refs, reads = [], []
+    for _ in range(30):
+        R = rand(150)
+        probes = [R[j:j + k] for j in range(0, 150 - T, s)]
+        A = R.copy()
+        last = (len(probes) - 1) * s
+        A[last + k // 2] = {65: 67, 67: 71, 71: 84, 84: 65}[int(A[last + k // 2])]
+        B = np.concatenate([np.concatenate([rand(int(rng.integers(20, 60))), probes[i]])
+                            for i in rng.permutation(len(probes))])
+        C = np.concatenate([R[:75], rand(3), R[75:]])
+        D = np.concatenate([R[:4 * s + k]] + [np.concatenate([rand(int(rng.integers(25, 50))), p])
+                                              for p in probes[5:]])
+        # P: R up to the end of probe window 4 plus a tail's worth, then
+        # random (one chain of 5); Q: R with a substitution inside window 4
+        # only (chains of 4 and 3, which cover more of R than P does).
+        R2 = rand(150)
+        P = np.concatenate([R2[:4 * s + k + s - 1], rand(20)])
+        Q = R2.copy()
+        x4 = 4 * s + k // 2 + 1
+        Q[x4] = {65: 67, 67: 71, 71: 84, 84: 65}[int(Q[x4])]
+        reads.append(R2)
+        for x in (A, B, C, D, P, Q):
+            refs.append(np.concatenate([rand(int(rng.integers(200, 600))), x,
+                                        rand(int(rng.integers(200, 600)))]))
+        reads.append(R)
+        reads.append(np.concatenate([R[:60], rand(1), R[61:]]))
+    ro = np.zeros(len(refs) + 1, np.uint64)
+    ro[1:] = np.cumsum([len(r) for r in refs])
+    qo = np.zeros(len(reads) + 1, np.uint64)
+    qo[1:] = np.cumsum([len(r) for r in reads])
+    return np.concatenate(refs), ro, np.concatenate(reads), qo
 
 
 def write_fasta(path, prefix, by, off):
@@ -147,9 +251,16 @@ def main():
     for name, T, rspec, qspec, seed, edits in datasets():
         if only and name not in only:
             continue
-        rb, ro = D.make_refs(rspec, seed)
-        qb, qo, truth = D.make_reads(qspec, rb, ro, seed)
-        apply_edits(edits, rb, ro, qb, qo, np.random.default_rng(seed))
+        if edits == "engineered":
+            rb, ro, qb, qo = engineered(seed, T)
+        else:
+            rb, ro = D.make_refs(rspec, seed)
+            rng = np.random.default_rng(seed)
+            if edits != "n_runs":
+                apply_edits(edits, rb, ro, None, None, rng)
+            qb, qo, truth = D.make_reads(qspec, rb, ro, seed)
+            if edits == "n_runs":
+                apply_edits(edits, rb, ro, qb, qo, rng)
         base = os.path.join(RUNS, name)
         os.makedirs(base, exist_ok=True)
         write_fasta(os.path.join(base, "reference.fasta"), "r", rb, ro)
