@@ -482,38 +482,79 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
 
   uint32_t nP = 0, nranks = 0, nchains = 0, G = 0;
   bool direct = false;
-  if (!(c.flags & MH_CLASSIFY_NO_DIRECT) && nhits == nprobe && E == nhits && nhits >= 2) {
-    // Direct hit: every probe found a single-entry bucket, all entries on one
-    // diagonal of one reference.  The filters then keep everything and the
-    // hits form one chain, so the sorts are skipped.
-    const uint64_t r_ref = ws.k1[0] >> 32;
-    const uint32_t diag = ws.v1[0] - ws.hit_j[0];
-    int ok = 1;
-    for (uint32_t e = tid; e < E; e += BS)
-      ok &= ((ws.k1[e] >> 32) == r_ref) && (ws.v1[e] - ws.hit_j[e] == diag);
-    direct = __syncthreads_and(ok) != 0;
-    if (direct) {
-      for (uint32_t e = tid; e < E; e += BS) {
-        ws.k2[e] = (uint64_t)ws.v1[e];  // rank 0
-        ws.v2[e] = ws.hit_j[e];
+  if (!(c.flags & MH_CLASSIFY_NO_DIRECT) && nhits == nprobe && nhits >= 3) {
+    // Direct hit.  Every probe found a bucket, and one reference r* has
+    // exactly one entry in each bucket, all on one diagonal.  Then r* has the
+    // longest run N = nhits, the cut is N - 1, and if no other reference
+    // occurs in every bucket 1..N-2 (needed for a run of N - 1), r* is the
+    // only kept reference: its N pairs form one chain, G = N, one seed.  The
+    // sorts and scans are skipped; the result is the same as the full path.
+    const uint32_t N = nhits;
+    const uint32_t b0_lo = ws.hit_base[0], b0_hi = ws.hit_base[1];
+    uint32_t found = 0, rstar = 0;
+    for (uint32_t e = b0_lo + tid; e < b0_hi; e += BS) {
+      const uint32_t r = (uint32_t)(ws.k1[e] >> 32);
+      const uint32_t diag = ws.v1[e];  // hit 0 has query position 0
+      bool ok = true;
+      for (uint32_t i = 0; i < N && ok; i++) {
+        uint32_t cnt = 0;
+        bool on = false;
+        for (uint32_t f = ws.hit_base[i]; f < ws.hit_base[i + 1]; f++)
+          if ((uint32_t)(ws.k1[f] >> 32) == r) {
+            cnt++;
+            on = ws.v1[f] - ws.hit_j[i] == diag;
+          }
+        ok = cnt == 1 && on;
       }
-      if (tid == 0) {
-        ws.rref[0] = (uint32_t)r_ref;
-        ws.t3[0] = E;                      // chain maximum of rank 0
-        ws.v1[0] = 0;                      // chains of rank 0: [0, 1)
-        ws.v1[1] = 1;
-        ((uint32_t *)ws.k1)[0] = 0;        // chain 0: entries [0, E-1]
-        ((uint32_t *)ws.k1)[1] = E - 1;
-        atomicAdd(&c.out.ctr->direct, 1ull);
+      if (ok) {
+        found++;
+        rstar = r;
       }
-      nP = E;
-      nranks = 1;
-      nchains = 1;
-      G = E;
+    }
+    const uint32_t nfound = block_sum<BS>(found, ws.red);
+    if (nfound == 1) {
+      const uint32_t rs_ = block_max<BS>(found ? rstar : 0u, ws.red);
+      // Any other reference with a run of N - 1 occurs in buckets 1..N-2.
+      int other = 0;
+      for (uint32_t e = ws.hit_base[1] + tid; e < ws.hit_base[2]; e += BS) {
+        const uint32_t r = (uint32_t)(ws.k1[e] >> 32);
+        if (r == rs_) continue;
+        bool all = true;
+        for (uint32_t i = 2; i + 1 < N && all; i++) {
+          bool in = false;
+          for (uint32_t f = ws.hit_base[i]; f < ws.hit_base[i + 1] && !in; f++)
+            in = (uint32_t)(ws.k1[f] >> 32) == r;
+          all = in;
+        }
+        other |= all ? 1 : 0;
+      }
+      direct = __syncthreads_or(other) == 0;
+      if (direct) {
+        // P = r*'s entries in hit order (already sorted by refPos, qPos).
+        if (tid == 0) {
+          uint32_t t = 0;
+          for (uint32_t i = 0; i < N; i++)
+            for (uint32_t f = ws.hit_base[i]; f < ws.hit_base[i + 1]; f++)
+              if ((uint32_t)(ws.k1[f] >> 32) == rs_) {
+                ws.k2[t] = (uint64_t)ws.v1[f];  // rank 0
+                ws.v2[t] = ws.hit_j[i];
+                t++;
+              }
+          ws.rref[0] = rs_;
+          ws.t3[0] = N;                      // chain maximum of rank 0
+          ws.v1[0] = 0;                      // chains of rank 0 start at chain 0
+          ((uint32_t *)ws.k1)[0] = 0;        // chain 0 spans pairs [0, N-1]
+          ((uint32_t *)ws.k1)[1] = N - 1;
+          atomicAdd(&c.out.ctr->direct, 1ull);
+        }
+        nP = N;
+        nranks = 1;
+        nchains = 1;
+        G = N;
+      }
       __syncthreads();
     }
   }
-
 
   if (!direct) {
     // Successive-occurrence filter.  Sort by (ref, hit) and measure runs of
