@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""Benchmark of the read-classification path on one MI355X.
+
+Workload: 5,000 references (log-normal lengths, median 50 kb, sigma 1.0, capped
+at 2 Mb, GC 40-60 %), k = 16 (T = 30), 1,000,000 reads x 150 bp (85 % sampled
+with 2 % substitutions, 5 % exact substrings, 10 % random), seeded synthetic
+data.  A step classifies the whole batch (both strands of every read: probe,
+filters, chains, scoring, best hit, strand merge) with the reads already in
+HBM.  Prints one JSON line; progress goes to stderr.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def device_tensors(torch, qb, qo):
+    d_reads = torch.from_numpy(qb).to("cuda")
+    d_off = torch.from_numpy(qo.view(np.int64)).to("cuda")
+    return d_reads, d_off
+
+
+def run_steps(torch, handle, ix, d_reads, d_off, n, flags, steps, warmup, native):
+    """Warm-up, then `steps` timed classify calls bracketed by synchronisation.
+    Returns (seconds per step, per-step kernel timings, counters)."""
+    d_out = torch.empty(n * native.READ_RESULT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    top_cap = 16 * n + 1024
+    d_top = torch.empty(top_cap * 4, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.ExternalStream(handle.stream_ptr())
+    for _ in range(warmup):
+        tn, _ = ix.classify_device(d_reads, d_off, n, d_out, d_top, top_cap, flags=flags)
+        assert tn <= top_cap
+    torch.cuda.synchronize()
+    timings = []
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        ix.classify_device(d_reads, d_off, n, d_out, d_top, top_cap,
+                           flags=flags | native.MH_CLASSIFY_TIMING, wait=False)
+        timings.append(handle.timing())
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_s = ev0.elapsed_time(ev1) / 1000.0
+    counters = handle.counters()
+    out = d_out.cpu().numpy().view(native.READ_RESULT_DTYPE)
+    top = d_top.cpu().numpy().view(np.uint32)
+    return max(wall, ev_s) / steps, timings, counters, out, top
+
+
+def cpu_baseline(rb, ro, qb, qo, threshold, sample, threads, gpu_out, gpu_top):
+    """The oracle (C++ restatement of the reference CPU path) on a bounded
+    sample of the same reads; also checks the GPU results on that sample."""
+    from oracle import oracle as O
+    from mirage_amd import native
+    t = time.perf_counter()
+    oix = O.Index(rb, ro, threshold)
+    build_s = time.perf_counter() - t
+    n = min(sample, len(qo) - 1)
+    sq = qb[: int(qo[n])]
+    so = qo[: n + 1]
+    oix.classify(sq, so, n_threads=threads)  # warm-up
+    best = None
+    for _ in range(3):
+        t = time.perf_counter()
+        out, top, _ = oix.classify(sq, so, n_threads=threads)
+        dt = time.perf_counter() - t
+        best = dt if best is None else min(best, dt)
+    g = gpu_out[:n]
+    same = (np.array_equal(g["highest"].view(np.uint32), out["highest"].view(np.uint32))
+            and np.array_equal(g["strand"], out["strand"])
+            and native.tie_sets(g, gpu_top) == O.top_sets(out, top))
+    return {"reads_per_s": n / best, "seconds": best, "reads": n, "index_build_s": build_s,
+            "bit_exact": bool(same)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the 1M reads")
+    ap.add_argument("--cpu-sample", type=int, default=400_000)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--mixed-reads", type=int, default=200_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the classification path on a "
+              "single device and has no multi-GPU mode (--gpus must be 1).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from mirage_amd import datagen as D
+    from mirage_amd import native
+
+    w = D.workload(2, args.scale)
+    t = time.perf_counter()
+    rb, ro, qb, qo, truth = D.generate(w)
+    n = len(qo) - 1
+    log(f"data: {len(ro) - 1} refs, {len(rb) / 1e6:.1f} Mbp, {n} reads "
+        f"({time.perf_counter() - t:.1f} s)")
+    handle = native.Handle(0, 32768)
+    t = time.perf_counter()
+    ix = handle.build_index(rb, ro, w.threshold)
+    torch.cuda.synchronize()
+    index_s = time.perf_counter() - t
+    info = ix.info()
+    log(f"device index: {info['n_entries']} entries, {info['device_bytes'] / 2**30:.2f} GiB, "
+        f"{index_s:.2f} s")
+    d_reads, d_off = device_tensors(torch, qb, qo)
+
+    log("batched: warm-up + timed steps")
+    sec, timings, ctr, out, top = run_steps(torch, handle, ix, d_reads, d_off, n,
+                                            native.MH_CLASSIFY_BATCHED, args.steps, args.warmup,
+                                            native)
+    batched_rps = n / sec
+    log(f"batched: {sec * 1e3:.3f} ms/step, {batched_rps / 1e6:.2f} M reads/s")
+
+    plain = None
+    if not args.no_plain:
+        psec, ptim, _, pout, ptop = run_steps(torch, handle, ix, d_reads, d_off, n, 0, args.steps,
+                                              args.warmup, native)
+        same = (np.array_equal(pout["highest"].view(np.uint32), out["highest"].view(np.uint32))
+                and native.tie_sets(pout, ptop) == native.tie_sets(out, top))
+        plain = {"reads_per_s": n / psec, "ms_per_step": psec * 1e3, "same_results": bool(same),
+                 "kernel_ms": float(np.mean([t_["bucket_ms"][0] for t_ in ptim]))}
+        log(f"plain: {psec * 1e3:.3f} ms/step, {n / psec / 1e6:.2f} M reads/s")
+
+    mixed = None
+    if not args.no_mixed:
+        wm = D.workload(3)
+        wm.reads.n = args.mixed_reads
+        mqb, mqo, _ = D.make_reads(wm.reads, rb, ro, wm.seed)
+        md_reads, md_off = device_tensors(torch, mqb, mqo)
+        nm = len(mqo) - 1
+        bs, _, _, bo, bt = run_steps(torch, handle, ix, md_reads, md_off, nm,
+                                     native.MH_CLASSIFY_BATCHED, max(2, args.steps // 2), 1, native)
+        ps, _, _, po, pt = run_steps(torch, handle, ix, md_reads, md_off, nm, 0,
+                                     max(2, args.steps // 2), 1, native)
+        same = (np.array_equal(bo["highest"].view(np.uint32), po["highest"].view(np.uint32))
+                and native.tie_sets(bo, bt) == native.tie_sets(po, pt))
+        mixed = {"reads": nm, "bases": int(mqo[-1]), "batched_reads_per_s": nm / bs,
+                 "plain_reads_per_s": nm / ps, "speedup": ps / bs, "same_results": bool(same)}
+        log(f"mixed-length: batched {nm / bs / 1e6:.3f} M reads/s, plain {nm / ps / 1e6:.3f} "
+            f"M reads/s, x{ps / bs:.2f}")
+        del md_reads, md_off
+
+    # Roofline of the dominant kernel: the LDS kernel of the 150 bp bucket.
+    # Algorithmic bytes per launch: a 16-byte bucket record per probe, 8 bytes
+    # per bucket entry read, 2 bytes per extension comparison, the read bases
+    # of both strands, and a 32-byte result per strand.
+    kernel_ms = float(np.mean([t_["bucket_ms"][0] for t_ in timings]))
+    alg_bytes = (16 * ctr["probes"] + 8 * ctr["entries"] + 2 * ctr["ext_compares"] +
+                 2 * int(qo[-1]) + 32 * ctr["queries"])
+    achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "k_classify<64> (reads <= 320 bp)", "kernel_ms": round(kernel_ms, 4),
+                "alg_bytes_per_launch": alg_bytes}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: oracle on {min(args.cpu_sample, n)} reads, {threads} threads")
+        cb = cpu_baseline(rb, ro, qb, qo, w.threshold, args.cpu_sample, threads, out, top)
+        cpu = {"value": round(cb["reads_per_s"], 1), "unit": "reads/s", "cores": threads,
+               "kind": "port",
+               "sample": f"first {cb['reads']} reads of the same workload against the full "
+                         f"index (oracle index build {cb['index_build_s']:.1f} s, not timed)",
+               "bit_exact_vs_gpu": cb["bit_exact"]}
+        log(f"cpu: {cb['reads_per_s'] / 1e3:.1f} k reads/s, gpu results on the sample "
+            f"bit-exact: {cb['bit_exact']}")
+
+    line = {
+        "metric": "reads classified/sec on 1 MI355X, batched+served vs plain; p50/p99 latency",
+        "value": round(batched_rps, 1),
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded)",
+        "config": {"workload": "5000 refs lognormal(50 kb, 1.0) GC 40-60%, k=16 (T=30), "
+                               f"{n} reads x 150 bp, 2% subs, 5% exact, 10% random",
+                   "mode": "batched (length-bucketed persistent kernels)",
+                   "reads": n, "index_entries": info["n_entries"],
+                   "index_build_s": round(index_s, 3)},
+        "probes_per_s": round(ctr["probes"] / sec, 1),
+        "direct_hit_strands": ctr["direct"],
+        "plain": plain,
+        "mixed_length": mixed,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -53,6 +53,7 @@ typedef struct mh_index mh_index;
 #define MH_CLASSIFY_ULTRAFAST 1u   /* no head/tail extension (MatrixCPU.cpp:69) */
 #define MH_CLASSIFY_NO_DIRECT 2u   /* disable the direct-hit fast path */
 #define MH_CLASSIFY_BATCHED 4u     /* length-bucketed persistent kernels */
+#define MH_CLASSIFY_TIMING 8u      /* record kernel timings (mh_get_timing) */
 
 /* Per-read result after the strand merge (MirAge/Analyser.cpp:592-611).
  * Top-tie reference ids (ascending) are top_pool[top_off, top_off+n_top).
@@ -105,7 +106,17 @@ typedef struct {
   uint64_t cands;          /* seeded candidates */
   uint64_t direct;         /* strands resolved by the direct-hit fast path */
   uint64_t overflow;       /* strands sent to the large workspace */
+  uint64_t ext_compares;   /* base comparisons of head/tail extension */
 } mh_counters;
+
+/* Kernel timings of the last call made with MH_CLASSIFY_TIMING. */
+typedef struct {
+  float total_ms;          /* first to last kernel of the call */
+  float bucket_ms[8];      /* LDS kernel per length bucket (plain form: [0]) */
+  float big_ms;            /* large-workspace kernel */
+  float merge_ms;          /* strand merge */
+  uint32_t n_buckets;
+} mh_timing;
 
 /* ---- handle ------------------------------------------------------------ */
 
@@ -163,6 +174,8 @@ mh_status mh_classify_host(mh_handle *h, const mh_index *idx, const uint8_t *h_r
 
 /* Counters of the last mh_classify / mh_classify_host call (synchronises). */
 mh_status mh_get_counters(mh_handle *h, mh_counters *out);
+/* Kernel timings of the last call made with MH_CLASSIFY_TIMING (synchronises). */
+mh_status mh_get_timing(mh_handle *h, mh_timing *out);
 
 /* Scoring only, the role CUDAGetHighestScore plays in the reference
  * (MatrixGPUCU.cu:108-467 with host :571-1969): score given seeds.

@@ -28,6 +28,7 @@ mh_status exec_init(mh_handle *h, Exec &x) {
   MH_HIP(h, hipHostMalloc(&x.h_ctr, sizeof(DevCounters), hipHostMallocDefault));
   MH_HIP(h, hipMemsetAsync(x.d_ctr, 0, sizeof(DevCounters), x.stream));
   std::memset(x.h_ctr, 0, sizeof(DevCounters));
+  for (auto &e : x.ev) MH_HIP(h, hipEventCreate(&e));
   return MH_OK;
 }
 
@@ -36,6 +37,8 @@ void exec_free(Exec &x) {
   if (x.d_scratch) hipFree(x.d_scratch);
   if (x.d_ctr) hipFree(x.d_ctr);
   if (x.h_ctr) hipHostFree(x.h_ctr);
+  for (auto &e : x.ev)
+    if (e) hipEventDestroy(e);
   if (x.stream) hipStreamDestroy(x.stream);
   x = Exec();
 }
@@ -254,6 +257,12 @@ static mh_status classify_on(mh_handle *h, Exec &x, const mh_index *idx, const u
   MH_HIP(h, hipStreamSynchronize(x.stream));
   if (top_needed) *top_needed = x.h_ctr->top_used;
   if (cand_needed) *cand_needed = x.h_ctr->cand_used;
+  if (x.h_ctr->tie_used > (unsigned long long)x.tie_factor * 2ull * n_reads + 256) {
+    // Tie sets larger than the scratch pool: grow it for the next call.
+    while ((unsigned long long)x.tie_factor * 2ull * n_reads + 256 < x.h_ctr->tie_used)
+      x.tie_factor *= 2;
+    return set_error(h, MH_ERR_OVERFLOW, "tie pool grown; call again");
+  }
   if (x.h_ctr->top_used > a.out.top_cap || (d_cands && x.h_ctr->cand_used > cand_cap))
     return set_error(h, MH_ERR_OVERFLOW, "output pool too small");
   return MH_OK;
@@ -349,6 +358,31 @@ mh_status mh_get_counters(mh_handle *h, mh_counters *out) {
   MH_HIP(h, hipStreamSynchronize(h->ex.stream));
   counters_from(*h->ex.h_ctr, h->last.queries / 2, h->last);
   *out = h->last;
+  return MH_OK;
+}
+
+mh_status mh_get_timing(mh_handle *h, mh_timing *out) {
+  if (!h || !out) return MH_ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->device);
+  std::memset(out, 0, sizeof(*out));
+  Exec &x = h->ex;
+  if (!x.timed) return set_error(h, MH_ERR_STATE, "last call was not timed");
+  MH_HIP(h, hipEventSynchronize(x.ev[11]));
+  auto ms = [&](int a, int b) {
+    float v = 0;
+    hipEventElapsedTime(&v, x.ev[a], x.ev[b]);
+    return v;
+  };
+  out->total_ms = ms(0, 11);
+  out->n_buckets = x.timed_buckets;
+  int prev = 9;
+  for (uint32_t i = 0; i < x.timed_buckets; i++) {
+    out->bucket_ms[x.bucket_order[i]] = ms(prev, 1 + (int)i);
+    prev = 1 + (int)i;
+  }
+  out->big_ms = ms(prev, 10);
+  out->merge_ms = ms(10, 11);
   return MH_OK;
 }
 

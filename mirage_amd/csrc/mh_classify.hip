@@ -26,6 +26,7 @@
 // to an overflow list and finished by a 1024-thread kernel whose workspace
 // lives in HBM.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cstring>
@@ -134,14 +135,40 @@ __device__ __forceinline__ bool pair_less(uint64_t ka, uint32_t va, uint64_t kb,
   return ka < kb || (ka == kb && va < vb);
 }
 
-// Ascending bitonic sort of (key, val) pairs over n (power of two) slots.
+// Ascending sort of (key, val) pairs over n (power of two) slots.  Up to 64
+// slots are ranked inside the first wavefront (each lane counts the smaller
+// elements through shuffles and writes its element to its rank); larger
+// arrays use a bitonic network in LDS.
 template <int BS>
 __device__ void bitonic_sort(uint64_t *key, uint32_t *val, uint32_t n) {
+  if (n <= 64) {
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      const uint32_t lane = threadIdx.x;
+      uint64_t k = lane < n ? key[lane] : ~0ull;
+      uint32_t v = lane < n ? val[lane] : ~0u;
+      uint32_t rank = 0;
+      for (uint32_t u = 0; u < n; u++) {
+        const uint32_t klo = __shfl((uint32_t)k, u, 64), khi = __shfl((uint32_t)(k >> 32), u, 64);
+        const uint32_t vu = __shfl(v, u, 64);
+        const uint64_t ku = ((uint64_t)khi << 32) | klo;
+        rank += (pair_less(ku, vu, k, v) || (ku == k && vu == v && u < lane)) ? 1u : 0u;
+      }
+      __builtin_amdgcn_wave_barrier();
+      if (lane < n) {
+        key[rank] = k;
+        val[rank] = v;
+      }
+    }
+    __syncthreads();
+    return;
+  }
   for (uint32_t size = 2; size <= n; size <<= 1) {
     for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
       __syncthreads();
+      const uint32_t lg = 31 - __clz(stride);
       for (uint32_t t = threadIdx.x; t < (n >> 1); t += BS) {
-        const uint32_t i = 2 * stride * (t / stride) + (t % stride);
+        const uint32_t i = ((t >> lg) << (lg + 1)) + (t & (stride - 1));
         const uint32_t j = i + stride;
         const bool asc = (i & size) == 0;
         const uint64_t ka = key[i], kb = key[j];
@@ -177,6 +204,7 @@ struct Ws {
   uint32_t *bitmap;    // qcap / 32 + 1
   uint32_t *red;       // 64 words
   unsigned long long *acc;  // 2
+  unsigned long long *stats;  // kNumStats, per workgroup
 };
 
 __host__ __device__ inline size_t al8(size_t x) { return (x + 7) & ~size_t(7); }
@@ -188,7 +216,7 @@ __host__ __device__ inline size_t ws_bytes(const Caps &c) {
   b += 2 * (al8(c.ecap * 8ull) + al8(c.ecap * 4ull));
   b += 4 * al8(c.ecap * 4ull);
   b += al8((c.qcap / 32 + 1) * 4ull);
-  b += 64 * 4 + 16;
+  b += 64 * 4 + 16 + 64;
   return b;
 }
 
@@ -215,6 +243,7 @@ __device__ inline Ws ws_carve(unsigned char *base, const Caps &c) {
   w.bitmap = (uint32_t *)take((c.qcap / 32 + 1) * 4ull);
   w.red = (uint32_t *)take(64 * 4);
   w.acc = (unsigned long long *)take(16);
+  w.stats = (unsigned long long *)take(64);
   return w;
 }
 
@@ -256,10 +285,11 @@ __device__ __forceinline__ float matrix_score(uint64_t sum_r, uint64_t sum_c, ui
 __device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const uint8_t *rb,
                                             uint64_t ref_len, uint32_t sref, uint32_t rs,
                                             uint32_t lref, uint32_t re, int T, bool ultrafast,
-                                            uint32_t &tot, uint32_t &head) {
+                                            uint32_t &tot, uint32_t &head, uint32_t &compares) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t mid = re - rs;
   uint32_t t = 0, h = 0;
+  compares = 0;
   if (!ultrafast) {
     // Tail: at most min(T, ref_len - lref) bases, inside the query.
     uint64_t tl = (uint64_t)lref < ref_len ? ref_len - lref : 0;
@@ -276,6 +306,8 @@ __device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const 
         break;
       }
     }
+    // the reference compares until the first mismatch or the limit
+    compares = t + (t < tl ? 1u : 0u);
     // Head: query row rs-1-l against reference byte sref-l; the base that
     // meets a boundary is not counted.
     if (rs > 0) {
@@ -292,6 +324,7 @@ __device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const 
             break;
           }
         }
+        compares += h + 1;
       }
     }
   }
@@ -327,14 +360,15 @@ __device__ void score_ref(const uint8_t *q, uint32_t m, const uint8_t *rb, uint6
     ws.acc[1] = 0;
   }
   __syncthreads();
-  unsigned long long sr = 0, sc = 0;
+  unsigned long long sr = 0, sc = 0, ncmp = 0;
   for (uint32_t i = wid; i < n_seeds; i += NW) {
     uint32_t sref, sq, lref, lq;
     get_seed(i, sref, sq, lref, lq);
     const uint32_t rs = sq, re = lq;
     if (rs > re) continue;  // MatrixCPU.cpp:57-58
-    uint32_t tot, h;
-    extend_seed(q, m, rb, ref_len, sref, rs, lref, re, T, ultrafast, tot, h);
+    uint32_t tot, h, cmp;
+    extend_seed(q, m, rb, ref_len, sref, rs, lref, re, T, ultrafast, tot, h, cmp);
+    ncmp += cmp;
     if (tot >= (uint32_t)T) {  // MatrixCPU.cpp:178-184
       sc += tot;
       sr += (unsigned long long)tot * (tot + 1) / 2;
@@ -345,6 +379,7 @@ __device__ void score_ref(const uint8_t *q, uint32_t m, const uint8_t *rb, uint6
     atomicAdd(&ws.acc[0], sr);
     atomicAdd(&ws.acc[1], sc);
   }
+  if (lane == 0 && ncmp && ws.stats) atomicAdd(&ws.stats[kExt], ncmp);
   __syncthreads();
   uint32_t pc = 0;
   for (uint32_t w = threadIdx.x; w < nw; w += BS) pc += __popc(ws.bitmap[w]);
@@ -374,9 +409,11 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
   QueryOut res;
   res.highest = -1.0f;
   res.n_top = 0;
-  res.top_off = 0;
+  for (int i = 0; i < kTopInline; i++) res.top_ref[i] = 0;
+  res.tie_off = 0;
   res.n_cand = 0;
   res.status = 0;
+  res.pad = 0;
   if (m64 > cap.qcap) {
     if (!last_resort) return kOverflow;
     res.status = MH_READ_TOO_LONG;
@@ -457,9 +494,9 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
     return kDone;
   }
   if (tid == 0) {
-    atomicAdd(&c.out.ctr->probes, (unsigned long long)nprobe);
-    atomicAdd(&c.out.ctr->hits, (unsigned long long)nhits);
-    atomicAdd(&c.out.ctr->entries, (unsigned long long)E);
+    ws.stats[kProbes] += nprobe;
+    ws.stats[kHits] += nhits;
+    ws.stats[kEntries] += E;
   }
   if (nhits == 0 || E == 0) {  // Analyser.cpp:1989-1990, :2012-2013
     write_query(c, q, res);
@@ -545,7 +582,7 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
           ws.v1[0] = 0;                      // chains of rank 0 start at chain 0
           ((uint32_t *)ws.k1)[0] = 0;        // chain 0 spans pairs [0, N-1]
           ((uint32_t *)ws.k1)[1] = N - 1;
-          atomicAdd(&c.out.ctr->direct, 1ull);
+          ws.stats[kDirect] += 1;
         }
         nP = N;
         nranks = 1;
@@ -553,6 +590,27 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
         G = N;
       }
       __syncthreads();
+    }
+  }
+
+  if (!direct) {
+    // A seed needs two pairs of one reference one stride apart on both axes
+    // (a chain link).  Links only join pairs of consecutive probes, so when
+    // no entry of a hit has its successor in the next probe's bucket, every
+    // chain has length 1 whatever the filters keep, and the query gets no
+    // seeds (Analyser.cpp:2398-2403): stop before the sorts.
+    int link = 0;
+    for (uint32_t e = tid; e < E; e += BS) {
+      const uint32_t i = (uint32_t)ws.k1[e];
+      if (i + 1 < nhits && ws.hit_j[i + 1] == ws.hit_j[i] + (uint32_t)s) {
+        const uint32_t r = (uint32_t)(ws.k1[e] >> 32), p = ws.v1[e] + (uint32_t)s;
+        for (uint32_t f = ws.hit_base[i + 1]; f < ws.hit_base[i + 2]; f++)
+          link |= ((uint32_t)(ws.k1[f] >> 32) == r && ws.v1[f] == p) ? 1 : 0;
+      }
+    }
+    if (!__syncthreads_or(link)) {
+      write_query(c, q, res);
+      return kDone;
     }
   }
 
@@ -767,8 +825,8 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
   }
   __syncthreads();
   if (tid == 0) {
-    atomicAdd(&c.out.ctr->seeds, (unsigned long long)seeds_total);
-    atomicAdd(&c.out.ctr->cands, (unsigned long long)seeded);
+    ws.stats[kSeeds] += seeds_total;
+    ws.stats[kCands] += seeded;
   }
 
   // Best hit: first strict maximum of the relative scores (all >= 0), and
@@ -794,20 +852,19 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
       ntop += tot;
     }
     __syncthreads();
-    if (tid == 0) {
-      const unsigned long long off = atomicAdd(&c.out.ctr->top_used, (unsigned long long)ntop);
-      ws.red[40] = (uint32_t)off;
-      ws.red[41] = (off + ntop <= c.out.top_cap) ? 1u : 0u;
-    }
-    __syncthreads();
-    const uint32_t off = ws.red[40];
-    if (ws.red[41]) {
-      for (uint32_t i = tid; i < ntop; i += BS) c.out.top[off + i] = ws.t3[i];
-      res.n_top = ntop;
-      res.top_off = off;
+    res.n_top = ntop;
+    if (ntop <= (uint32_t)kTopInline) {
+      for (uint32_t i = 0; i < ntop; i++) res.top_ref[i] = ws.t3[i];
     } else {
-      res.n_top = ntop;
-      res.top_off = 0xFFFFFFFFu;  // pool exhausted; the host reports the size needed
+      if (tid == 0) {
+        const unsigned long long off = atomicAdd(&c.out.ctr->tie_used, (unsigned long long)ntop);
+        ws.red[40] = (uint32_t)off;
+        ws.red[41] = (off + ntop <= c.out.tie_cap) ? 1u : 0u;
+      }
+      __syncthreads();
+      res.tie_off = ws.red[40];
+      if (ws.red[41])
+        for (uint32_t i = tid; i < ntop; i += BS) c.out.ties[res.tie_off + i] = ws.t3[i];
     }
   }
   write_query(c, q, res);
@@ -815,7 +872,21 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
   return kDone;
 }
 
-// One query per workgroup (plain form) or a persistent loop over a queue.
+__device__ __forceinline__ void stats_begin(const Ws &ws) {
+  if (threadIdx.x < kNumStats) ws.stats[threadIdx.x] = 0;
+  __syncthreads();
+}
+
+__device__ __forceinline__ void stats_flush(const Ws &ws, DevCounters *ctr) {
+  __syncthreads();
+  if (threadIdx.x < kNumStats && ws.stats[threadIdx.x])
+    atomicAdd(&ctr->stats[blockIdx.x & (kStatSlots - 1)][threadIdx.x], ws.stats[threadIdx.x]);
+}
+
+constexpr uint32_t kQueueChunk = 8;
+
+// One query per workgroup (plain form) or a persistent loop that takes
+// queries from a queue kQueueChunk at a time.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_classify(Ctx c, Caps cap, const uint32_t *qlist,
                                                  const unsigned long long *qstart, uint32_t nq,
@@ -824,32 +895,39 @@ __global__ __launch_bounds__(BS) void k_classify(Ctx c, Caps cap, const uint32_t
   extern __shared__ __align__(16) unsigned char smem[];
   const Ws ws = ws_carve(smem, cap);
   if (qlist && qstart) qlist += *qstart;
+  stats_begin(ws);
   if (head == nullptr) {
     const uint32_t i = blockIdx.x;
-    if (i >= nq) return;
-    const uint32_t q = qlist ? qlist[i] : i;
-    if (process_query<BS>(c, ws, cap, q, false) == kOverflow && threadIdx.x == 0) {
-      const unsigned long long slot = atomicAdd(&c.out.ctr->overflow_n, 1ull);
-      if (slot < c.out.overflow_cap) c.out.overflow[slot] = q;
+    if (i < nq) {
+      const uint32_t q = qlist ? qlist[i] : i;
+      if (process_query<BS>(c, ws, cap, q, false) == kOverflow && threadIdx.x == 0) {
+        const unsigned long long slot = atomicAdd(&c.out.ctr->overflow_n, 1ull);
+        if (slot < c.out.overflow_cap) c.out.overflow[slot] = q;
+      }
     }
+    stats_flush(ws, c.out.ctr);
     return;
   }
   const unsigned long long n = nq_dev ? *nq_dev : nq;
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) {
-      const unsigned long long i = atomicAdd(head, 1ull);
+      const unsigned long long i = atomicAdd(head, (unsigned long long)kQueueChunk);
       ws.red[48] = i < n ? (uint32_t)i : 0xFFFFFFFFu;
     }
     __syncthreads();
-    const uint32_t i = ws.red[48];
-    if (i == 0xFFFFFFFFu) break;
-    const uint32_t q = qlist ? qlist[i] : i;
-    if (process_query<BS>(c, ws, cap, q, false) == kOverflow && threadIdx.x == 0) {
-      const unsigned long long slot = atomicAdd(&c.out.ctr->overflow_n, 1ull);
-      if (slot < c.out.overflow_cap) c.out.overflow[slot] = q;
+    const uint32_t base = ws.red[48];
+    if (base == 0xFFFFFFFFu) break;
+    const uint32_t end = (uint32_t)min((unsigned long long)base + kQueueChunk, n);
+    for (uint32_t i = base; i < end; i++) {
+      const uint32_t q = qlist ? qlist[i] : i;
+      if (process_query<BS>(c, ws, cap, q, false) == kOverflow && threadIdx.x == 0) {
+        const unsigned long long slot = atomicAdd(&c.out.ctr->overflow_n, 1ull);
+        if (slot < c.out.overflow_cap) c.out.overflow[slot] = q;
+      }
     }
   }
+  stats_flush(ws, c.out.ctr);
 }
 
 // Queries that overflowed the LDS kernels, with the workspace in HBM.
@@ -857,6 +935,7 @@ __global__ __launch_bounds__(1024) void k_classify_big(Ctx c, Caps cap, unsigned
                                                        size_t ws_stride) {
   const Ws ws = ws_carve(scratch + blockIdx.x * ws_stride, cap);
   __shared__ uint32_t slot;
+  stats_begin(ws);
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -868,15 +947,19 @@ __global__ __launch_bounds__(1024) void k_classify_big(Ctx c, Caps cap, unsigned
     if (slot == 0xFFFFFFFFu) break;
     process_query<1024>(c, ws, cap, c.out.overflow[slot], true);
   }
+  stats_flush(ws, c.out.ctr);
 }
 
 // Strand merge (Analyser.cpp:592-611): the reverse strand wins only when its
 // HighestScore is strictly higher; a read is reported only with candidates
-// and a non-zero HighestScore (Analyser.cpp:388).
-__global__ void k_merge(const QueryOut *qo, uint32_t n_reads, mh_read_result *out) {
+// and a non-zero HighestScore (Analyser.cpp:388).  Pass 1 writes the result
+// and the number of reported ids; an exclusive scan gives each read its
+// place in the top-tie pool; pass 2 copies the ids.
+__global__ void k_merge_count(const QueryOut *qo, uint32_t n_reads, mh_read_result *out,
+                              uint32_t *cnt) {
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n_reads;
        r += gridDim.x * blockDim.x) {
-    const QueryOut f = qo[2 * r], b = qo[2 * r + 1];
+    const QueryOut &f = qo[2 * r], &b = qo[2 * r + 1];
     const bool rc = f.highest < b.highest;
     const QueryOut &w = rc ? b : f;
     mh_read_result o;
@@ -885,10 +968,35 @@ __global__ void k_merge(const QueryOut *qo, uint32_t n_reads, mh_read_result *ou
     o.n_cand = w.n_cand;
     const bool report = w.n_cand > 0 && w.highest != 0.0f;
     o.n_top = report ? w.n_top : 0u;
-    o.top_off = report ? w.top_off : 0u;
+    o.top_off = 0;
     o.status = f.status | b.status;
     out[r] = o;
+    cnt[r] = o.n_top;
   }
+}
+
+__global__ void k_merge_write(const QueryOut *qo, uint32_t n_reads, const uint32_t *off,
+                              const uint32_t *ties, uint64_t tie_cap, mh_read_result *out,
+                              uint32_t *top, uint64_t top_cap, DevCounters *ctr) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n_reads;
+       r += gridDim.x * blockDim.x) {
+    const uint32_t n = out[r].n_top;
+    const uint32_t o = off[r];
+    out[r].top_off = o;
+    if (!n || (uint64_t)o + n > top_cap) continue;
+    const QueryOut &w = qo[2 * r + out[r].strand];
+    if (n <= (uint32_t)kTopInline) {
+      for (uint32_t i = 0; i < n; i++) top[o + i] = w.top_ref[i];
+    } else if ((uint64_t)w.tie_off + n <= tie_cap) {
+      for (uint32_t i = 0; i < n; i++) top[o + i] = ties[w.tie_off + i];
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctr->top_used = off[n_reads];
+}
+
+__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = 0;
 }
 
 __global__ void k_zero(unsigned long long *p, uint32_t n) {
@@ -896,29 +1004,48 @@ __global__ void k_zero(unsigned long long *p, uint32_t n) {
     p[i] = 0;
 }
 
-// Length buckets for the batched form: bucket of each query by read length.
+// Length buckets for the batched form: bucket of each read by its length.
+// One read per thread; counts are gathered in LDS and added once per block.
+__device__ __forceinline__ uint32_t bucket_of(const Reads &rd, uint32_t r, const uint32_t *max_len,
+                                              uint32_t nb) {
+  const uint64_t m = rd.off[r + 1] - rd.off[r];
+  uint32_t b = 0;
+  while (b + 1 < nb && m > max_len[b]) b++;
+  return b;
+}
+
 __global__ void k_bin_count(Reads rd, const uint32_t *max_len, uint32_t nb,
                             unsigned long long *cnt) {
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < rd.n; r += gridDim.x * blockDim.x) {
-    const uint64_t m = rd.off[r + 1] - rd.off[r];
-    uint32_t b = 0;
-    while (b + 1 < nb && m > max_len[b]) b++;
-    atomicAdd(&cnt[b], 2ull);
-  }
+  __shared__ unsigned int loc[8];
+  if (threadIdx.x < 8) loc[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < rd.n) atomicAdd(&loc[bucket_of(rd, r, max_len, nb)], 2u);
+  __syncthreads();
+  if (threadIdx.x < nb && loc[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], (unsigned long long)loc[threadIdx.x]);
 }
 
 __global__ void k_bin_scatter(Reads rd, const uint32_t *max_len, uint32_t nb,
-                              const unsigned long long *cnt, unsigned long long *cursor,
+                              const unsigned long long *start, unsigned long long *cursor,
                               uint32_t *list) {
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < rd.n; r += gridDim.x * blockDim.x) {
-    const uint64_t m = rd.off[r + 1] - rd.off[r];
-    uint32_t b = 0;
-    while (b + 1 < nb && m > max_len[b]) b++;
-    unsigned long long base = 0;
-    for (uint32_t i = 0; i < b; i++) base += cnt[i];
-    const unsigned long long at = atomicAdd(&cursor[b], 2ull);
-    list[base + at] = 2 * r;
-    list[base + at + 1] = 2 * r + 1;
+  __shared__ unsigned int loc[8];
+  __shared__ unsigned long long base[8];
+  if (threadIdx.x < 8) loc[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t b = 0, idx = 0;
+  if (r < rd.n) {
+    b = bucket_of(rd, r, max_len, nb);
+    idx = atomicAdd(&loc[b], 2u);
+  }
+  __syncthreads();
+  if (threadIdx.x < nb && loc[threadIdx.x])
+    base[threadIdx.x] = start[threadIdx.x] +
+                        atomicAdd(&cursor[threadIdx.x], (unsigned long long)loc[threadIdx.x]);
+  __syncthreads();
+  if (r < rd.n) {
+    list[base[b] + idx] = 2 * r;
+    list[base[b] + idx + 1] = 2 * r + 1;
   }
 }
 
@@ -947,6 +1074,7 @@ __global__ __launch_bounds__(64) void k_score_seeds(IndexView ix, Luts lut, cons
   ws.bitmap = (uint32_t *)(smem + al8(qcap));
   ws.red = ws.bitmap + (qcap / 32 + 1);
   ws.acc = (unsigned long long *)(ws.red + 64);
+  ws.stats = nullptr;
   for (uint32_t p = blockIdx.x; p < n_pairs; p += gridDim.x) {
     const uint32_t qi = pq[p], R = pr[p];
     const uint64_t q0 = qoff[qi];
@@ -1024,6 +1152,20 @@ hipError_t launch_tile(const mh_tile &t, const Ctx &c, const Caps &cap, const ui
   }
 }
 
+// Workgroups of one LDS kernel resident per CU at this LDS size.
+uint32_t resident_blocks(uint32_t block, size_t lds) {
+  int n = 0;
+  hipError_t e = hipErrorInvalidValue;
+  switch (block) {
+    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_classify<64>, 64, lds); break;
+    case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_classify<128>, 128, lds); break;
+    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_classify<256>, 256, lds); break;
+    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_classify<512>, 512, lds); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_classify<1024>, 1024, lds);
+  }
+  return (e == hipSuccess && n > 0) ? (uint32_t)n : 1u;
+}
+
 __global__ void k_max_len(Reads rd, unsigned long long *out) {
   uint32_t mx = 0;
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < rd.n; r += gridDim.x * blockDim.x)
@@ -1065,17 +1207,45 @@ std::vector<mh_tile> default_tiles(uint32_t max_read_len) {
 
 // Scratch layout for one classify call (bytes from the handle's arena):
 //   QueryOut[2n] | overflow list u32[2n] | binned list u32[2n] |
-//   bucket words (u64[64]) | large-kernel workspaces
-size_t classify_scratch_bytes(uint32_t n_reads, uint32_t max_read_len, int s) {
+//   merge counts u32[n+1] | merge offsets u32[n+1] | tie pool u32[f*2n] |
+//   bucket words u64[64] | scan temp | large-kernel workspaces
+struct Layout {
+  size_t qo, overflow, binned, cnt, off, ties, bw, scan, big, total;
+  size_t scan_bytes, tie_cap;
+};
+
+Layout layout(uint32_t n_reads, uint32_t max_read_len, int s, uint32_t tie_factor) {
   const size_t nq = 2ull * n_reads;
+  Layout L;
+  size_t o = 0;
+  auto take = [&](size_t b) {
+    size_t at = o;
+    o = (o + b + 255) & ~size_t(255);
+    return at;
+  };
+  L.qo = take(nq * sizeof(QueryOut));
+  L.overflow = take(nq * 4);
+  L.binned = take(nq * 4);
+  L.cnt = take((n_reads + 1ull) * 4);
+  L.off = take((n_reads + 1ull) * 4);
+  L.tie_cap = (size_t)tie_factor * nq + 256;
+  L.ties = take(L.tie_cap * 4);
+  L.bw = take(64 * 8);
+  L.scan_bytes = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, L.scan_bytes, (const uint32_t *)nullptr,
+                                   (uint32_t *)nullptr, (int)(n_reads + 1));
+  L.scan = take(L.scan_bytes);
   Caps big;
   big.qcap = (max_read_len + 63) & ~63u;
   big.pcap = big.qcap / (uint32_t)(s > 0 ? s : 1) + 2;
   big.ecap = kBigEcap;
-  size_t b = al8(nq * sizeof(QueryOut)) + 2 * al8(nq * 4) + 64 * 8;
-  b = (b + 255) & ~size_t(255);
-  b += kBigBlocks * ((ws_bytes(big) + 255) & ~size_t(255));
-  return b;
+  L.big = take(kBigBlocks * ((ws_bytes(big) + 255) & ~size_t(255)));
+  L.total = o;
+  return L;
+}
+
+size_t classify_scratch_bytes(uint32_t n_reads, uint32_t max_read_len, int s) {
+  return layout(n_reads, max_read_len, s, 4).total;
 }
 
 mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool batched) {
@@ -1083,21 +1253,19 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
   if (e != hipSuccess) return hip_error(h, e, "complement table");
   const uint32_t n = a.rd.n;
   const uint64_t nq = 2ull * n;
-  const size_t need = classify_scratch_bytes(n, h->max_read_len, a.ix.s);
-  mh_status st = ensure_scratch(h, x, need);
+  const Layout L = layout(n, h->max_read_len, a.ix.s, x.tie_factor);
+  mh_status st = ensure_scratch(h, x, L.total);
   if (st) return st;
   unsigned char *base = (unsigned char *)x.d_scratch;
-  size_t o = 0;
-  QueryOut *qo = (QueryOut *)(base + o);
-  o += al8(nq * sizeof(QueryOut));
-  uint32_t *overflow = (uint32_t *)(base + o);
-  o += al8(nq * 4);
-  uint32_t *binned = (uint32_t *)(base + o);
-  o += al8(nq * 4);
-  unsigned long long *bw = (unsigned long long *)(base + o);  // [0..7] cnt, [8..15] cursor, [16..23] start
-  o += 64 * 8;
-  o = (o + 255) & ~size_t(255);
-  unsigned char *big_ws = base + o;
+  QueryOut *qo = (QueryOut *)(base + L.qo);
+  uint32_t *overflow = (uint32_t *)(base + L.overflow);
+  uint32_t *binned = (uint32_t *)(base + L.binned);
+  uint32_t *mcnt = (uint32_t *)(base + L.cnt);
+  uint32_t *moff = (uint32_t *)(base + L.off);
+  // bucket words: [0..7] count, [8..15] cursor, [16..23] start, [24] longest
+  // read, [32..35] bucket length limits (u32)
+  unsigned long long *bw = (unsigned long long *)(base + L.bw);
+  unsigned char *big_ws = base + L.big;
 
   Ctx c;
   c.ix = a.ix;
@@ -1109,17 +1277,24 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
   c.out.ctr = x.d_ctr;
   c.out.overflow = overflow;
   c.out.overflow_cap = (uint32_t)nq;
+  c.out.ties = (uint32_t *)(base + L.ties);
+  c.out.tie_cap = L.tie_cap;
   hipStream_t s = x.stream;
   const uint32_t zn = sizeof(DevCounters) / 8;
-  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, (unsigned long long *)x.d_ctr, zn);
+  const bool timed = (a.flags & MH_CLASSIFY_TIMING) != 0;
+  x.timed = false;
+  x.timed_buckets = 0;
+  if (timed) hipEventRecord(x.ev[0], s);
+  hipLaunchKernelGGL(k_zero, dim3(8), dim3(256), 0, s, (unsigned long long *)x.d_ctr, zn);
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, bw, 64u);
   if (nq) {
     std::vector<mh_tile> tiles = h->tiles.empty() ? default_tiles(h->max_read_len) : h->tiles;
+    const unsigned gr = (unsigned)((n + 255) / 256);
     if (!batched) {
       // Plain form: one launch per batch, one workgroup per query, with the
       // single tile that covers the longest read of the batch.
-      hipLaunchKernelGGL(k_max_len, dim3(std::min<uint32_t>((n + 255) / 256, 1024)), dim3(256),
-                         0, s, a.rd, bw + 24);
+      hipLaunchKernelGGL(k_max_len, dim3(std::min<uint32_t>(gr, 1024)), dim3(256), 0, s, a.rd,
+                         bw + 24);
       unsigned long long mlen = 0;
       e = hipMemcpyAsync(&mlen, bw + 24, 8, hipMemcpyDeviceToHost, s);
       if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -1128,37 +1303,42 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
       while (ti + 1 < tiles.size() && mlen > tiles[ti].max_len) ti++;
       const mh_tile &t0 = tiles[ti];
       const Caps cap = caps_for(t0, a.ix.s, h->lds_per_block);
+      if (timed) hipEventRecord(x.ev[9], s);
       e = launch_tile(t0, c, cap, nullptr, nullptr, (uint32_t)nq, nullptr, nullptr, (uint32_t)nq,
                       s);
       if (e != hipSuccess) return hip_error(h, e, "k_classify");
+      if (timed) {
+        hipEventRecord(x.ev[1], s);
+        x.bucket_order[0] = 0;
+        x.timed_buckets = 1;
+      }
     } else {
       const uint32_t nb = (uint32_t)std::min<size_t>(tiles.size(), 8);
-      uint32_t *d_maxlen = (uint32_t *)(bw + 32);  // 8 words inside the bucket area
+      uint32_t *d_maxlen = (uint32_t *)(bw + 32);
       uint32_t ml[8];
       for (uint32_t i = 0; i < nb; i++) ml[i] = tiles[i].max_len;
       e = hipMemcpyAsync(d_maxlen, ml, nb * 4, hipMemcpyHostToDevice, s);
       if (e != hipSuccess) return hip_error(h, e, "tile lengths");
       const Reads rd = a.rd;
-      hipLaunchKernelGGL(k_bin_count, dim3(std::min<uint32_t>((n + 255) / 256, 4096)), dim3(256),
-                         0, s, rd, d_maxlen, nb, bw);
+      hipLaunchKernelGGL(k_bin_count, dim3(gr), dim3(256), 0, s, rd, d_maxlen, nb, bw);
       hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, bw, nb, bw + 16);
-      hipLaunchKernelGGL(k_bin_scatter, dim3(std::min<uint32_t>((n + 255) / 256, 4096)),
-                         dim3(256), 0, s, rd, d_maxlen, nb, bw, bw + 8, binned);
+      hipLaunchKernelGGL(k_bin_scatter, dim3(gr), dim3(256), 0, s, rd, d_maxlen, nb, bw + 16,
+                         bw + 8, binned);
       // Long buckets first so that their tail overlaps with the short ones.
+      if (timed) hipEventRecord(x.ev[9], s);
       for (int b = (int)nb - 1; b >= 0; b--) {
         const mh_tile &t = tiles[b];
         const Caps cap = caps_for(t, a.ix.s, h->lds_per_block);
         uint32_t grid = t.grid;
-        if (!grid) {
-          const size_t lds = ws_bytes(cap);
-          const uint32_t per_cu = (uint32_t)std::max<size_t>(
-              1, std::min<size_t>(h->lds_per_block / std::max<size_t>(lds, 1), 2048 / t.block));
-          grid = h->cu_count * per_cu;
-        }
+        if (!grid) grid = h->cu_count * resident_blocks(t.block, ws_bytes(cap));
         // queue b: binned[start[b], start[b] + cnt[b]), head queue_head[b]
         e = launch_tile(t, c, cap, binned, bw + 16 + b, 0, bw + b, &x.d_ctr->queue_head[b],
                         grid, s);
         if (e != hipSuccess) return hip_error(h, e, "k_classify batched");
+        if (timed) {
+          hipEventRecord(x.ev[1 + x.timed_buckets], s);
+          x.bucket_order[x.timed_buckets++] = (uint32_t)b;
+        }
       }
     }
     Caps big;
@@ -1167,8 +1347,22 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
     big.ecap = kBigEcap;
     const size_t stride = (ws_bytes(big) + 255) & ~size_t(255);
     hipLaunchKernelGGL(k_classify_big, dim3(kBigBlocks), dim3(1024), 0, s, c, big, big_ws, stride);
-    hipLaunchKernelGGL(k_merge, dim3(std::min<uint32_t>((n + 255) / 256, 4096)), dim3(256), 0, s,
-                       (const QueryOut *)qo, n, a.final_out);
+    if (timed) hipEventRecord(x.ev[10], s);
+    hipLaunchKernelGGL(k_merge_count, dim3(std::min<uint32_t>(gr, 4096)), dim3(256), 0, s,
+                       (const QueryOut *)qo, n, a.final_out, mcnt);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, mcnt + n, 1u);
+    size_t sb = L.scan_bytes;
+    e = hipcub::DeviceScan::ExclusiveSum((void *)(base + L.scan), sb, mcnt, moff, (int)(n + 1),
+                                         s);
+    if (e != hipSuccess) return hip_error(h, e, "merge scan");
+    hipLaunchKernelGGL(k_merge_write, dim3(std::min<uint32_t>(gr, 4096)), dim3(256), 0, s,
+                       (const QueryOut *)qo, n, (const uint32_t *)moff,
+                       (const uint32_t *)c.out.ties, (uint64_t)c.out.tie_cap, a.final_out,
+                       a.out.top, a.out.top_cap, x.d_ctr);
+    if (timed) {
+      hipEventRecord(x.ev[11], s);
+      x.timed = true;
+    }
     e = hipGetLastError();
     if (e != hipSuccess) return hip_error(h, e, "classify launch");
   }
@@ -1176,14 +1370,18 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
 }
 
 void counters_from(const DevCounters &d, uint64_t n_reads, mh_counters &o) {
+  unsigned long long t[kNumStats] = {};
+  for (int sl = 0; sl < kStatSlots; sl++)
+    for (int i = 0; i < kNumStats; i++) t[i] += d.stats[sl][i];
   o.queries = 2 * n_reads;
-  o.probes = d.probes;
-  o.hits = d.hits;
-  o.entries = d.entries;
-  o.seeds = d.seeds;
-  o.cands = d.cands;
-  o.direct = d.direct;
+  o.probes = t[kProbes];
+  o.hits = t[kHits];
+  o.entries = t[kEntries];
+  o.seeds = t[kSeeds];
+  o.cands = t[kCands];
+  o.direct = t[kDirect];
   o.overflow = d.overflow_n;
+  o.ext_compares = t[kExt];
 }
 
 mh_status launch_score_seeds(mh_handle *h, const IndexView &ix, const Luts &luts,

@@ -44,24 +44,34 @@ struct Reads {
   uint32_t n;
 };
 
-// Per read strand (query q = 2 * read + strand).
+// Per read strand (query q = 2 * read + strand).  Up to kTopInline tied
+// references are kept inline; larger tie sets go to the tie pool.
+constexpr int kTopInline = 4;
 struct QueryOut {
   float highest;
   uint32_t n_top;
-  uint32_t top_off;
+  uint32_t top_ref[kTopInline];
+  uint32_t tie_off;
   uint32_t n_cand;
   uint32_t status;
-  uint32_t pad[3];
+  uint32_t pad;
 };
+
+// Work statistics, accumulated per workgroup and added to one of
+// kStatSlots slots (spread over cache lines) when the workgroup ends.
+enum Stat { kProbes = 0, kHits, kEntries, kSeeds, kCands, kDirect, kExt, kNumStats };
+constexpr int kStatSlots = 256;
 
 // Device counters, zeroed before each classify call.
 struct DevCounters {
-  unsigned long long top_used;
+  unsigned long long top_used;    // final top-tie pool fill (set by the merge)
   unsigned long long cand_used;
   unsigned long long overflow_n;
   unsigned long long big_overflow_n;
-  unsigned long long probes, hits, entries, seeds, cands, direct;
+  unsigned long long tie_used;    // scratch tie pool fill
+  unsigned long long pad[3];
   unsigned long long queue_head[16];
+  unsigned long long stats[kStatSlots][8];
 };
 
 struct Outputs {
@@ -73,6 +83,8 @@ struct Outputs {
   DevCounters *ctr;
   uint32_t *overflow;      // queries deferred to the large workspace
   uint32_t overflow_cap;
+  uint32_t *ties;          // tie pool for tie sets larger than kTopInline
+  uint64_t tie_cap;
 };
 
 struct Caps {
@@ -121,6 +133,13 @@ struct Exec {
   size_t scratch_bytes = 0;
   DevCounters *d_ctr = nullptr;
   DevCounters *h_ctr = nullptr;  // pinned host copy
+  // timing events: [0] start, [1..8] after bucket kernels, [9] before the
+  // bucket kernels, [10] after the large kernel, [11] after the merge
+  hipEvent_t ev[12] = {};
+  uint32_t tie_factor = 4;       // tie pool entries per query, grown on overflow
+  bool timed = false;
+  uint32_t timed_buckets = 0;
+  uint32_t bucket_order[8] = {};
 };
 }  // namespace mh
 

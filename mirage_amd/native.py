@@ -23,7 +23,9 @@ READ_RESULT_DTYPE = np.dtype([("highest", "<f4"), ("strand", "<u4"), ("n_top", "
 CAND_DTYPE = np.dtype([("read", "<u4"), ("strand", "<u4"), ("ref", "<u4"), ("hits", "<u4"),
                        ("sum_r", "<u8"), ("sum_c", "<u8"), ("score", "<f4"), ("rel", "<f4"),
                        ("n_seeds", "<u4"), ("pad", "<u4")])
-COUNTER_FIELDS = ("queries", "probes", "hits", "entries", "seeds", "cands", "direct", "overflow")
+COUNTER_FIELDS = ("queries", "probes", "hits", "entries", "seeds", "cands", "direct", "overflow",
+                  "ext_compares")
+MH_CLASSIFY_TIMING = 8
 
 
 class IndexInfo(ctypes.Structure):
@@ -35,6 +37,12 @@ class IndexInfo(ctypes.Structure):
 
 class Counters(ctypes.Structure):
     _fields_ = [(f, ctypes.c_uint64) for f in COUNTER_FIELDS]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_float), ("bucket_ms", ctypes.c_float * 8),
+                ("big_ms", ctypes.c_float), ("merge_ms", ctypes.c_float),
+                ("n_buckets", ctypes.c_uint32)]
 
 
 class Tile(ctypes.Structure):
@@ -57,7 +65,7 @@ class RequestInfo(ctypes.Structure):
 # Every symbol declared in include/mirage_hip.h.
 EXPORTS = ("mh_open", "mh_close", "mh_last_error", "mh_stream", "mh_max_read_len",
            "mh_max_batch_reads", "mh_index_build", "mh_index_free", "mh_index_get_info",
-           "mh_index_bucket", "mh_classify", "mh_classify_host", "mh_get_counters",
+           "mh_index_bucket", "mh_classify", "mh_classify_host", "mh_get_counters", "mh_get_timing",
            "mh_score_seeds", "mh_set_tiles", "mh_get_tiles", "mh_autotune", "mh_server_start",
            "mh_server_submit", "mh_server_wait", "mh_server_stop")
 
@@ -94,6 +102,7 @@ def load():
         "mh_classify": (i32, [P, P, P, P, u32, u32, P, P, u64, P, u64, P, P]),
         "mh_classify_host": (i32, [P, P, P, P, u32, u32, P, P, u64, P, u64, P, P]),
         "mh_get_counters": (i32, [P, ctypes.POINTER(Counters)]),
+        "mh_get_timing": (i32, [P, ctypes.POINTER(Timing)]),
         "mh_score_seeds": (i32, [P, P, P, P, P, P, P, P, u32, u32, P, P, P, P, P]),
         "mh_set_tiles": (i32, [P, P, u32]),
         "mh_get_tiles": (i32, [P, P, u32, ctypes.POINTER(u32)]),
@@ -171,6 +180,12 @@ class Handle:
                                          ctypes.byref(n)), "mh_get_tiles")
         return [(arr[i].max_len, arr[i].block, arr[i].entries_cap, arr[i].grid)
                 for i in range(n.value)]
+
+    def timing(self):
+        t = Timing()
+        self.check(self.lib.mh_get_timing(self.h, ctypes.byref(t)), "mh_get_timing")
+        return {"total_ms": t.total_ms, "bucket_ms": list(t.bucket_ms), "big_ms": t.big_ms,
+                "merge_ms": t.merge_ms, "n_buckets": t.n_buckets}
 
     def counters(self):
         c = Counters()
