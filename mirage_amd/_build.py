@@ -18,12 +18,13 @@ HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-ffp-contract=off", "-fno-fast-math", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument", "-Wno-unused-result",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument", "-Wno-unused-result", "-Wno-unused-value",
 ]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off", "-fno-fast-math"]
 
-HIP_SOURCES = ["mh_index.hip", "mh_classify.hip", "mh_api.hip"]
-HOST_SOURCES = ["mh_serve.cpp", "mh_autotune.cpp", "matrix_hip.cpp"]
+HIP_SOURCES = ["mh_index.hip", "mh_classify.hip", "mh_api.hip", "mh_serve.hip",
+               "mh_autotune.hip"]
+HOST_SOURCES = ["matrix_hip.cpp"]
 
 
 def datagen_lib_path():
@@ -81,9 +82,25 @@ def build_hip(verbose=False):
     return out
 
 
+def shim_driver_path():
+    return os.path.join(LIBDIR, "shim_driver")
+
+
+def build_shim_driver():
+    """Test driver of the C++ call surface (tests/cpp/shim_driver.cpp)."""
+    src = os.path.join(ROOT, "tests", "cpp", "shim_driver.cpp")
+    out = shim_driver_path()
+    _run([HIPCC, *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", f"-I{INCLUDE}", f"-I{CSRC}", src,
+          "-o", out, f"-L{LIBDIR}", "-lmirage_hip", f"-Wl,-rpath,{LIBDIR}",
+          "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
 def build_all():
     build_datagen()
-    return build_hip()
+    out = build_hip()
+    build_shim_driver()
+    return out
 
 
 def clean():

@@ -187,6 +187,13 @@ class Handle:
         return {"total_ms": t.total_ms, "bucket_ms": list(t.bucket_ms), "big_ms": t.big_ms,
                 "merge_ms": t.merge_ms, "n_buckets": t.n_buckets}
 
+    def autotune(self, index, d_reads, d_read_off, n_reads, iters=3):
+        """Times tile candidates per length bucket on device-resident reads and
+        installs the fastest (mh_autotune).  Returns the tile table."""
+        self.check(self.lib.mh_autotune(self.h, index.ix, _dev_ptr(d_reads), _dev_ptr(d_read_off),
+                                        n_reads, iters), "mh_autotune")
+        return self.get_tiles()
+
     def counters(self):
         c = Counters()
         self.check(self.lib.mh_get_counters(self.h, ctypes.byref(c)), "mh_get_counters")
@@ -280,6 +287,61 @@ class DeviceIndex:
                              ctypes.byref(cn) if wait else None)
         self.handle.check(st, "mh_classify")
         return (tn.value, cn.value) if wait else None
+
+
+class Server:
+    """Serving loop over a resident index (mh_server_start)."""
+
+    def __init__(self, index, hbm_budget=0, max_inflight=0, max_batch_reads=0, flags=0):
+        self.index = index
+        self.lib = index.handle.lib
+        cfg = ServerConfig(hbm_budget, max_inflight, max_batch_reads, flags, 0)
+        s = ctypes.c_void_p()
+        index.handle.check(self.lib.mh_server_start(index.handle.h, index.ix, ctypes.byref(cfg),
+                                                    ctypes.byref(s)), "mh_server_start")
+        self.s = s
+        self.sizes = {}
+
+    def submit(self, reads, read_off):
+        reads = np.ascontiguousarray(reads, dtype=np.uint8)
+        read_off = np.ascontiguousarray(read_off, dtype=np.uint64)
+        rid = ctypes.c_uint64()
+        st = self.lib.mh_server_submit(self.s, _np_ptr(reads), _np_ptr(read_off),
+                                       len(read_off) - 1, ctypes.byref(rid))
+        if st != MH_OK:
+            raise MirageError(st, "mh_server_submit")
+        self.sizes[rid.value] = len(read_off) - 1
+        return rid.value
+
+    def wait(self, rid, timeout=-1.0):
+        """Returns (results, top pool, info dict) of request rid."""
+        n = self.sizes.pop(rid)
+        out = np.zeros(n, READ_RESULT_DTYPE)
+        cap = max(64, 8 * n)
+        while True:
+            top = np.zeros(cap, np.uint32)
+            need = ctypes.c_uint64()
+            info = RequestInfo()
+            st = self.lib.mh_server_wait(self.s, rid, timeout, _np_ptr(out), _np_ptr(top), cap,
+                                         ctypes.byref(need), ctypes.byref(info))
+            if st == MH_ERR_OVERFLOW:
+                cap = int(need.value) + 16
+                continue
+            if st != MH_OK:
+                raise MirageError(st, f"request {rid}")
+            return out, top[:need.value], {"submit_s": info.submit_s, "admit_s": info.admit_s,
+                                           "done_s": info.done_s, "n_reads": info.n_reads}
+
+    def stop(self):
+        if self.s:
+            self.lib.mh_server_stop(self.s)
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass
 
 
 def tie_sets(out, top):

@@ -1,0 +1,80 @@
+"""Serving loop and autotuner on the GPU: results are the same alone and in
+the mix, and tile choices do not change results."""
+import numpy as np
+import pytest
+
+from mirage_amd import datagen as D
+from mirage_amd import native
+
+pytestmark = pytest.mark.gpu
+
+
+def split(qb, qo, sizes):
+    reqs, r = [], 0
+    for n in sizes:
+        a, b = int(qo[r]), int(qo[r + n])
+        reqs.append((qb[a:b], qo[r:r + n + 1] - qo[r]))
+        r += n
+    return reqs
+
+
+@pytest.fixture(scope="module")
+def served(handle):
+    w = D.workload(4, 0.005)  # 100 refs ~10 Mbp, reads 100-5000 bp
+    rb, ro, qb, qo, _ = D.generate(w)
+    ix = handle.build_index(rb, ro, w.threshold)
+    return w, ix, qb, qo
+
+
+def test_each_request_same_alone_and_in_mix(served):
+    w, ix, qb, qo = served
+    rng = np.random.default_rng(1)
+    sizes = []
+    left = len(qo) - 1
+    while left > 0:
+        n = int(min(left, max(1, 10 ** rng.uniform(0, 2.7))))
+        sizes.append(n)
+        left -= n
+    reqs = split(qb, qo, sizes)
+    alone = [ix.classify_host(b, o, flags=native.MH_CLASSIFY_BATCHED) for b, o in reqs]
+    srv = native.Server(ix, max_inflight=3, max_batch_reads=256)
+    ids = [srv.submit(b, o) for b, o in reqs]
+    for (aout, atop, _), rid in zip(alone, ids):
+        out, top, info = srv.wait(rid)
+        assert np.array_equal(out["highest"].view(np.uint32), aout["highest"].view(np.uint32))
+        assert np.array_equal(out["strand"], aout["strand"])
+        assert np.array_equal(out["n_cand"], aout["n_cand"])
+        assert native.tie_sets(out, top) == native.tie_sets(aout, atop)
+        assert info["submit_s"] <= info["admit_s"] <= info["done_s"]
+    srv.stop()
+
+
+def test_server_admits_while_busy(served):
+    """Requests submitted while batches run are admitted into later batches."""
+    w, ix, qb, qo = served
+    reqs = split(qb, qo, [50] * 20)
+    srv = native.Server(ix, max_inflight=2, max_batch_reads=100)
+    ids = [srv.submit(b, o) for b, o in reqs]
+    infos = [srv.wait(r)[2] for r in ids]
+    srv.stop()
+    admits = sorted(i["admit_s"] for i in infos)
+    assert admits[-1] > admits[0]  # not all in one batch
+
+
+def test_autotune_keeps_results(handle):
+    import torch
+    w = D.workload(3, 0.001)
+    w.refs.n = 200
+    rb, ro, qb, qo, _ = D.generate(w)
+    ix = handle.build_index(rb, ro, w.threshold)
+    before, btop, _ = ix.classify_host(qb, qo, flags=native.MH_CLASSIFY_BATCHED)
+    saved = handle.get_tiles()
+    d_reads = torch.from_numpy(qb).cuda()
+    d_off = torch.from_numpy(qo.view(np.int64)).cuda()
+    torch.cuda.synchronize()
+    tiles = handle.autotune(ix, d_reads, d_off, len(qo) - 1, iters=1)
+    assert len(tiles) == len(saved)
+    after, atop, _ = ix.classify_host(qb, qo, flags=native.MH_CLASSIFY_BATCHED)
+    handle.set_tiles(saved)
+    assert np.array_equal(before["highest"].view(np.uint32), after["highest"].view(np.uint32))
+    assert native.tie_sets(before, btop) == native.tie_sets(after, atop)
