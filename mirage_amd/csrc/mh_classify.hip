@@ -1069,11 +1069,17 @@ __global__ __launch_bounds__(64) void k_score_seeds(IndexView ix, Luts lut, cons
                                                    float *o_rel, uint64_t *o_sr, uint64_t *o_sc,
                                                    uint32_t *o_hits) {
   extern __shared__ __align__(16) unsigned char smem[];
+  // Every piece starts on an 8-byte boundary: the 64-bit LDS atomics and
+  // paired 64-bit accesses on acc need natural alignment.
   Ws ws;
+  size_t o = 0;
   ws.q = smem;
-  ws.bitmap = (uint32_t *)(smem + al8(qcap));
-  ws.red = ws.bitmap + (qcap / 32 + 1);
-  ws.acc = (unsigned long long *)(ws.red + 64);
+  o += al8(qcap);
+  ws.bitmap = (uint32_t *)(smem + o);
+  o += al8((qcap / 32 + 1) * 4ull);
+  ws.red = (uint32_t *)(smem + o);
+  o += 64 * 4;
+  ws.acc = (unsigned long long *)(smem + o);
   ws.stats = nullptr;
   for (uint32_t p = blockIdx.x; p < n_pairs; p += gridDim.x) {
     const uint32_t qi = pq[p], R = pr[p];
@@ -1391,7 +1397,7 @@ mh_status launch_score_seeds(mh_handle *h, const IndexView &ix, const Luts &luts
                              uint64_t *d_sum_r, uint64_t *d_sum_c, uint32_t *d_hits) {
   if (!n_pairs) return MH_OK;
   const uint32_t qcap = (h->max_read_len + 63) & ~63u;
-  const size_t lds = al8(qcap) + (qcap / 32 + 1) * 4 + 64 * 4 + 16;
+  const size_t lds = al8(qcap) + al8((qcap / 32 + 1) * 4ull) + 64 * 4 + 16;
   const uint32_t grid = std::min<uint32_t>(n_pairs, 65536);
   hipLaunchKernelGGL(k_score_seeds, dim3(grid), dim3(64), lds, h->stream, ix, luts, d_q, d_qoff,
                      d_pq, d_pr, d_soff, d_seeds, n_pairs, qcap, flags, d_score, d_rel, d_sum_r,

@@ -37,7 +37,7 @@ def test_each_request_same_alone_and_in_mix(served):
         left -= n
     reqs = split(qb, qo, sizes)
     alone = [ix.classify_host(b, o, flags=native.MH_CLASSIFY_BATCHED) for b, o in reqs]
-    srv = native.Server(ix, max_inflight=3, max_batch_reads=256)
+    srv = native.Server(ix, max_inflight=3, max_batch_reads=1024)
     ids = [srv.submit(b, o) for b, o in reqs]
     for (aout, atop, _), rid in zip(alone, ids):
         out, top, info = srv.wait(rid)
@@ -52,8 +52,8 @@ def test_each_request_same_alone_and_in_mix(served):
 def test_server_admits_while_busy(served):
     """Requests submitted while batches run are admitted into later batches."""
     w, ix, qb, qo = served
-    reqs = split(qb, qo, [50] * 20)
-    srv = native.Server(ix, max_inflight=2, max_batch_reads=100)
+    reqs = split(qb, qo, [25] * 20)
+    srv = native.Server(ix, max_inflight=2, max_batch_reads=50)
     ids = [srv.submit(b, o) for b, o in reqs]
     infos = [srv.wait(r)[2] for r in ids]
     srv.stop()
