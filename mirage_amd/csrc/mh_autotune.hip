@@ -1,9 +1,9 @@
 // Tile autotuner for the length-bucketed persistent form.
 //
 // For every length bucket of the handle's tile table, each candidate
-// (threads per workgroup, LDS entry capacity) whose workspace fits the LDS
-// limit and still holds the bucket's longest read is timed on the caller's
-// reads; the fastest candidate is installed.  Results do not depend on the
+// (threads per workgroup, LDS entry capacity) is timed on the caller's reads
+// (the whole call, so that work a small tile pushes to the overflow tiers is
+// counted); the fastest candidate is installed.  Results do not depend on the
 // tile: a query whose working set does not fit a tile's LDS goes to the
 // large-workspace kernel, so only speed changes.
 #include <hip/hip_runtime.h>
@@ -35,9 +35,9 @@ extern "C" mh_status mh_autotune(mh_handle *h, const mh_index *idx, const uint8_
     tiles = h->tiles.empty() ? default_tiles(h->max_read_len) : h->tiles;
   }
   const uint32_t blocks[] = {64, 128, 256, 512};
-  const uint32_t ecaps[] = {128, 256, 512, 1024, 2048, 4096};
+  const uint32_t ecaps[] = {64, 128, 256, 512, 1024, 2048, 4096};
   mh_status st = MH_OK;
-  auto time_bucket = [&](const std::vector<mh_tile> &tt, uint32_t b, float &ms) -> mh_status {
+  auto time_bucket = [&](const std::vector<mh_tile> &tt, uint32_t, float &ms) -> mh_status {
     std::vector<float> v;
     for (uint32_t it = 0; it <= iters; it++) {
       std::lock_guard<std::mutex> lk(h->mu);
@@ -54,12 +54,9 @@ extern "C" mh_status mh_autotune(mh_handle *h, const mh_index *idx, const uint8_
       mh_status s2 = launch_classify(h, h->ex, a, true);
       if (s2) return s2;
       MH_HIP(h, hipEventSynchronize(h->ex.ev[11]));
-      // bucket b's kernel runs between the events around it
-      int pos = -1;
-      for (uint32_t i = 0; i < h->ex.timed_buckets; i++)
-        if (h->ex.bucket_order[i] == b) pos = (int)i;
+      // The whole call: a smaller tile can move work into the overflow tiers.
       float t = 0;
-      hipEventElapsedTime(&t, h->ex.ev[pos == 0 ? 9 : pos], h->ex.ev[1 + pos]);
+      hipEventElapsedTime(&t, h->ex.ev[0], h->ex.ev[11]);
       if (it) v.push_back(t);  // first call warms up
     }
     std::sort(v.begin(), v.end());

@@ -854,7 +854,9 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
     __syncthreads();
     res.n_top = ntop;
     if (ntop <= (uint32_t)kTopInline) {
-      for (uint32_t i = 0; i < ntop; i++) res.top_ref[i] = ws.t3[i];
+      // constant indices keep res in registers
+#pragma unroll
+      for (int i = 0; i < kTopInline; i++) res.top_ref[i] = (uint32_t)i < ntop ? ws.t3[i] : 0u;
     } else {
       if (tid == 0) {
         const unsigned long long off = atomicAdd(&c.out.ctr->tie_used, (unsigned long long)ntop);
@@ -901,7 +903,7 @@ __global__ __launch_bounds__(BS) void k_classify(Ctx c, Caps cap, const uint32_t
     if (i < nq) {
       const uint32_t q = qlist ? qlist[i] : i;
       if (process_query<BS>(c, ws, cap, q, false) == kOverflow && threadIdx.x == 0) {
-        const unsigned long long slot = atomicAdd(&c.out.ctr->overflow_n, 1ull);
+        const unsigned long long slot = atomicAdd(c.out.overflow_cnt, 1ull);
         if (slot < c.out.overflow_cap) c.out.overflow[slot] = q;
       }
     }
@@ -922,7 +924,7 @@ __global__ __launch_bounds__(BS) void k_classify(Ctx c, Caps cap, const uint32_t
     for (uint32_t i = base; i < end; i++) {
       const uint32_t q = qlist ? qlist[i] : i;
       if (process_query<BS>(c, ws, cap, q, false) == kOverflow && threadIdx.x == 0) {
-        const unsigned long long slot = atomicAdd(&c.out.ctr->overflow_n, 1ull);
+        const unsigned long long slot = atomicAdd(c.out.overflow_cnt, 1ull);
         if (slot < c.out.overflow_cap) c.out.overflow[slot] = q;
       }
     }
@@ -930,22 +932,23 @@ __global__ __launch_bounds__(BS) void k_classify(Ctx c, Caps cap, const uint32_t
   stats_flush(ws, c.out.ctr);
 }
 
-// Queries that overflowed the LDS kernels, with the workspace in HBM.
+// Queries that overflowed both LDS tiers, with the workspace in HBM.
 __global__ __launch_bounds__(1024) void k_classify_big(Ctx c, Caps cap, unsigned char *scratch,
-                                                       size_t ws_stride) {
+                                                       size_t ws_stride, const uint32_t *list,
+                                                       const unsigned long long *count) {
   const Ws ws = ws_carve(scratch + blockIdx.x * ws_stride, cap);
   __shared__ uint32_t slot;
   stats_begin(ws);
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) {
-      const unsigned long long n = c.out.ctr->overflow_n;
+      const unsigned long long n = *count;
       const unsigned long long i = atomicAdd(&c.out.ctr->big_overflow_n, 1ull);
       slot = (i < n && i < c.out.overflow_cap) ? (uint32_t)i : 0xFFFFFFFFu;
     }
     __syncthreads();
     if (slot == 0xFFFFFFFFu) break;
-    process_query<1024>(c, ws, cap, c.out.overflow[slot], true);
+    process_query<1024>(c, ws, cap, list[slot], true);
   }
   stats_flush(ws, c.out.ctr);
 }
@@ -1204,7 +1207,7 @@ constexpr uint32_t kBigEcap = 1u << 19;
 
 std::vector<mh_tile> default_tiles(uint32_t max_read_len) {
   std::vector<mh_tile> t;
-  t.push_back(mh_tile{320, 64, 256, 0});
+  t.push_back(mh_tile{320, 64, 64, 0});
   t.push_back(mh_tile{1024, 128, 512, 0});
   t.push_back(mh_tile{4096, 256, 2048, 0});
   t.push_back(mh_tile{std::max<uint32_t>(max_read_len, 4097), 256, 2048, 0});
@@ -1212,11 +1215,11 @@ std::vector<mh_tile> default_tiles(uint32_t max_read_len) {
 }
 
 // Scratch layout for one classify call (bytes from the handle's arena):
-//   QueryOut[2n] | overflow list u32[2n] | binned list u32[2n] |
+//   QueryOut[2n] | overflow lists u32[2n] x 2 | binned list u32[2n] |
 //   merge counts u32[n+1] | merge offsets u32[n+1] | tie pool u32[f*2n] |
 //   bucket words u64[64] | scan temp | large-kernel workspaces
 struct Layout {
-  size_t qo, overflow, binned, cnt, off, ties, bw, scan, big, total;
+  size_t qo, overflow, overflow2, binned, cnt, off, ties, bw, scan, big, total;
   size_t scan_bytes, tie_cap;
 };
 
@@ -1231,6 +1234,7 @@ Layout layout(uint32_t n_reads, uint32_t max_read_len, int s, uint32_t tie_facto
   };
   L.qo = take(nq * sizeof(QueryOut));
   L.overflow = take(nq * 4);
+  L.overflow2 = take(nq * 4);
   L.binned = take(nq * 4);
   L.cnt = take((n_reads + 1ull) * 4);
   L.off = take((n_reads + 1ull) * 4);
@@ -1265,6 +1269,7 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
   unsigned char *base = (unsigned char *)x.d_scratch;
   QueryOut *qo = (QueryOut *)(base + L.qo);
   uint32_t *overflow = (uint32_t *)(base + L.overflow);
+  uint32_t *overflow2 = (uint32_t *)(base + L.overflow2);
   uint32_t *binned = (uint32_t *)(base + L.binned);
   uint32_t *mcnt = (uint32_t *)(base + L.cnt);
   uint32_t *moff = (uint32_t *)(base + L.off);
@@ -1282,6 +1287,7 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
   c.out.q = qo;
   c.out.ctr = x.d_ctr;
   c.out.overflow = overflow;
+  c.out.overflow_cnt = &x.d_ctr->overflow_n;
   c.out.overflow_cap = (uint32_t)nq;
   c.out.ties = (uint32_t *)(base + L.ties);
   c.out.tie_cap = L.tie_cap;
@@ -1347,12 +1353,27 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
         }
       }
     }
+    // Middle tier: queries that overflowed their tile, in a 256-thread LDS
+    // kernel with the largest workspace that fits; what still overflows goes
+    // to the HBM workspace.
+    {
+      const mh_tile mid{h->max_read_len, 256, 4096, 0};
+      const Caps mcap = caps_for(mid, a.ix.s, h->lds_per_block);
+      Ctx cm = c;
+      cm.out.overflow = overflow2;
+      cm.out.overflow_cnt = &x.d_ctr->overflow2_n;
+      const uint32_t mgrid = h->cu_count * resident_blocks(256, ws_bytes(mcap));
+      e = launch_tile(mid, cm, mcap, overflow, nullptr, 0, &x.d_ctr->overflow_n,
+                      &x.d_ctr->mid_head, mgrid, s);
+      if (e != hipSuccess) return hip_error(h, e, "k_classify middle tier");
+    }
     Caps big;
     big.qcap = (h->max_read_len + 63) & ~63u;
     big.pcap = big.qcap / (uint32_t)(a.ix.s > 0 ? a.ix.s : 1) + 2;
     big.ecap = kBigEcap;
     const size_t stride = (ws_bytes(big) + 255) & ~size_t(255);
-    hipLaunchKernelGGL(k_classify_big, dim3(kBigBlocks), dim3(1024), 0, s, c, big, big_ws, stride);
+    hipLaunchKernelGGL(k_classify_big, dim3(kBigBlocks), dim3(1024), 0, s, c, big, big_ws, stride,
+                       (const uint32_t *)overflow2, (const unsigned long long *)&x.d_ctr->overflow2_n);
     if (timed) hipEventRecord(x.ev[10], s);
     hipLaunchKernelGGL(k_merge_count, dim3(std::min<uint32_t>(gr, 4096)), dim3(256), 0, s,
                        (const QueryOut *)qo, n, a.final_out, mcnt);
@@ -1386,7 +1407,7 @@ void counters_from(const DevCounters &d, uint64_t n_reads, mh_counters &o) {
   o.seeds = t[kSeeds];
   o.cands = t[kCands];
   o.direct = t[kDirect];
-  o.overflow = d.overflow_n;
+  o.overflow = d.overflow_n + d.overflow2_n;
   o.ext_compares = t[kExt];
 }
 

@@ -69,7 +69,9 @@ struct DevCounters {
   unsigned long long overflow_n;
   unsigned long long big_overflow_n;
   unsigned long long tie_used;    // scratch tie pool fill
-  unsigned long long pad[3];
+  unsigned long long overflow2_n; // queries the middle tier passes to the HBM workspace
+  unsigned long long mid_head;    // middle-tier queue head
+  unsigned long long pad;
   unsigned long long queue_head[16];
   unsigned long long stats[kStatSlots][8];
 };
@@ -81,7 +83,8 @@ struct Outputs {
   mh_cand *cands;
   uint64_t cand_cap;
   DevCounters *ctr;
-  uint32_t *overflow;      // queries deferred to the large workspace
+  uint32_t *overflow;      // where queries too large for this tier are appended
+  unsigned long long *overflow_cnt;
   uint32_t overflow_cap;
   uint32_t *ties;          // tie pool for tie sets larger than kTopInline
   uint64_t tie_cap;
