@@ -50,10 +50,12 @@ def run_steps(torch, handle, ix, d_reads, d_off, n, flags, steps, warmup, native
     ev1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     ev0.record(stream)
+    timed = not (flags & native.MH_CLASSIFY_GRAPH)
     for _ in range(steps):
         ix.classify_device(d_reads, d_off, n, d_out, d_top, top_cap,
-                           flags=flags | native.MH_CLASSIFY_TIMING, wait=False)
-        timings.append(handle.timing())
+                           flags=flags | (native.MH_CLASSIFY_TIMING if timed else 0), wait=False)
+        if timed:
+            timings.append(handle.timing())
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -62,6 +64,84 @@ def run_steps(torch, handle, ix, d_reads, d_off, n, flags, steps, warmup, native
     out = d_out.cpu().numpy().view(native.READ_RESULT_DTYPE)
     top = d_top.cpu().numpy().view(np.uint32)
     return max(wall, ev_s) / steps, timings, counters, out, top
+
+
+def serving_leg(native, D, scale, n_requests, loads, log):
+    """Served request stream (workload 4): 20,000 references ~2 Gbp (scaled),
+    k = 16; requests of floor(10^U(0,4)) reads, 100-5,000 bp, 2 % substitutions
+    + 0.5 % indels, Poisson arrivals.  Saturation throughput first, then the
+    given fractions of it as offered loads; latency is submit -> results
+    ready, measured by the server."""
+    import threading
+    w = D.workload(4, scale)
+    t = time.perf_counter()
+    rb, ro = D.make_refs(w.refs, w.seed)
+    pool = D.ReadSpec(n=300_000, lengths=D.LengthSpec(mode=2, a=100, b=5000), sub_rate=0.02,
+                      indel_rate=0.005)
+    qb, qo, _ = D.make_reads(pool, rb, ro, w.seed)
+    log(f"serving: {len(ro) - 1} refs, {len(rb) / 1e9:.2f} Gbp, read pool {len(qo) - 1} "
+        f"({time.perf_counter() - t:.1f} s)")
+    h = native.Handle(0, 8192)
+    t = time.perf_counter()
+    ix = h.build_index(rb, ro, w.threshold)
+    info = ix.info()
+    log(f"serving index: {info['n_entries']} entries, {info['device_bytes'] / 2**30:.1f} GiB, "
+        f"{time.perf_counter() - t:.1f} s")
+    rng = np.random.default_rng(w.seed)
+    sizes = np.floor(10 ** rng.uniform(0, 4, n_requests)).astype(int)
+    starts = rng.integers(0, len(qo) - 1 - sizes)
+    reqs = [(qb[int(qo[a]):int(qo[a + n])], qo[a:a + n + 1] - qo[a]) for a, n in zip(starts, sizes)]
+    mean_reads = float(sizes.mean())
+
+    def run(rate):
+        srv = native.Server(ix, hbm_budget=16 << 30, max_inflight=3, max_batch_reads=65536,
+                            flags=0)
+        ids, lat = [], []
+        waiter_done = threading.Event()
+
+        def waiter():
+            for i in range(len(reqs)):
+                while i >= len(ids):
+                    time.sleep(0.0002)
+                _, _, inf = srv.wait(ids[i])
+                lat.append((inf["done_s"] - inf["submit_s"], inf["submit_s"], inf["done_s"]))
+            waiter_done.set()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        gaps = rng.exponential(1.0 / rate, len(reqs)) if rate else np.zeros(len(reqs))
+        nxt = t0
+        for (b, o), g in zip(reqs, gaps):
+            nxt += g
+            d = nxt - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            ids.append(srv.submit(b, o))
+        th.join()
+        srv.stop()
+        l = np.array([x[0] for x in lat])
+        span = max(x[2] for x in lat) - min(x[1] for x in lat)
+        return {"reads_per_s": float(sizes.sum() / span), "p50_ms": float(np.percentile(l, 50) * 1e3),
+                "p99_ms": float(np.percentile(l, 99) * 1e3), "requests": len(reqs)}
+
+    sat = run(0)
+    log(f"serving saturation: {sat['reads_per_s'] / 1e6:.2f} M reads/s")
+    out = {"workload": f"{len(ro) - 1} refs {len(rb) / 1e9:.2f} Gbp k=16; requests of "
+                       f"floor(10^U(0,4)) reads (mean {mean_reads:.0f}), 100-5000 bp, "
+                       "2% subs + 0.5% indels, Poisson arrivals",
+           "saturation": sat, "loads": []}
+    for f in loads:
+        rate = f * sat["reads_per_s"] / mean_reads
+        r = run(rate)
+        r["offered_fraction"] = f
+        r["offered_requests_per_s"] = rate
+        out["loads"].append(r)
+        log(f"serving load {f:.2f}: {r['reads_per_s'] / 1e6:.2f} M reads/s, p50 "
+            f"{r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms")
+    ix.free()
+    h.close()
+    return out
 
 
 def cpu_baseline(rb, ro, qb, qo, threshold, sample, threads, gpu_out, gpu_top):
@@ -102,6 +182,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-reads", type=int, default=200_000)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-scale", type=float, default=1.0)
+    ap.add_argument("--serve-requests", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the classification path on a "
@@ -132,8 +215,16 @@ def main():
     sec, timings, ctr, out, top = run_steps(torch, handle, ix, d_reads, d_off, n,
                                             native.MH_CLASSIFY_BATCHED, args.steps, args.warmup,
                                             native)
-    batched_rps = n / sec
-    log(f"batched: {sec * 1e3:.3f} ms/step, {batched_rps / 1e6:.2f} M reads/s")
+    log(f"batched (eager launches): {sec * 1e3:.3f} ms/step, {n / sec / 1e6:.2f} M reads/s")
+    gsec, _, _, gout, gtop = run_steps(torch, handle, ix, d_reads, d_off, n,
+                                       native.MH_CLASSIFY_BATCHED | native.MH_CLASSIFY_GRAPH,
+                                       args.steps, args.warmup, native)
+    graph_same = (np.array_equal(gout["highest"].view(np.uint32), out["highest"].view(np.uint32))
+                  and native.tie_sets(gout, gtop) == native.tie_sets(out, top))
+    log(f"batched (hipGraph replay): {gsec * 1e3:.3f} ms/step, {n / gsec / 1e6:.2f} M reads/s, "
+        f"same results {graph_same}")
+    batched_rps = n / min(sec, gsec)
+    sec = min(sec, gsec)
 
     plain = None
     if not args.no_plain:
@@ -177,6 +268,11 @@ def main():
                 "kernel": "k_classify<64> (reads <= 320 bp)", "kernel_ms": round(kernel_ms, 4),
                 "alg_bytes_per_launch": alg_bytes}
 
+    serve = None
+    if not args.no_serve:
+        serve = serving_leg(native, D, args.serve_scale, args.serve_requests, (0.25, 0.5, 0.9),
+                            log)
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -210,8 +306,10 @@ def main():
                    "index_build_s": round(index_s, 3)},
         "probes_per_s": round(ctr["probes"] / sec, 1),
         "direct_hit_strands": ctr["direct"],
+        "graph_replay_same_results": bool(graph_same),
         "plain": plain,
         "mixed_length": mixed,
+        "served": serve,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -54,6 +54,8 @@ typedef struct mh_index mh_index;
 #define MH_CLASSIFY_NO_DIRECT 2u   /* disable the direct-hit fast path */
 #define MH_CLASSIFY_BATCHED 4u     /* length-bucketed persistent kernels */
 #define MH_CLASSIFY_TIMING 8u      /* record kernel timings (mh_get_timing) */
+#define MH_CLASSIFY_GRAPH 16u      /* batched form replayed from a captured hipGraph;
+                                      captured again when any argument changes */
 
 /* Per-read result after the strand merge (MirAge/Analyser.cpp:592-611).
  * Top-tie reference ids (ascending) are top_pool[top_off, top_off+n_top).

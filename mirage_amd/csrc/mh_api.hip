@@ -34,6 +34,7 @@ mh_status exec_init(mh_handle *h, Exec &x) {
 
 void exec_free(Exec &x) {
   if (x.stream) hipStreamSynchronize(x.stream);
+  if (x.graph) hipGraphExecDestroy(x.graph);
   if (x.d_scratch) hipFree(x.d_scratch);
   if (x.d_ctr) hipFree(x.d_ctr);
   if (x.h_ctr) hipHostFree(x.h_ctr);
@@ -56,6 +57,8 @@ mh_status ensure_scratch(mh_handle *h, Exec &x, size_t bytes) {
 }
 
 namespace {
+
+uint64_t tiles_version(const mh_handle *h) { return h->tiles_version; }
 
 // scoreMax of MatrixResult::GetRelativeScore (MatrixResult.cpp:23-24).
 float score_max(uint32_t L) {
@@ -142,6 +145,7 @@ mh_status mh_open(int32_t device, uint32_t max_read_len, mh_handle **out) {
     st = exec_init(h, h->ex);
     h->stream = h->ex.stream;
   }
+  if (!st && ensure_comp_table(device) != hipSuccess) st = MH_ERR_HIP;
   if (!st) st = build_luts(h);
   if (!st) {
     std::vector<mh_tile> t = default_tiles(h->max_read_len);
@@ -249,7 +253,47 @@ static mh_status classify_on(mh_handle *h, Exec &x, const mh_index *idx, const u
   a.out.cands = d_cands;
   a.out.cand_cap = d_cands ? cand_cap : 0;
   a.final_out = d_out;
-  mh_status st = launch_classify(h, x, a, (flags & MH_CLASSIFY_BATCHED) != 0);
+  mh_status st;
+  if (flags & MH_CLASSIFY_GRAPH) {
+    // Replay the captured batched call when every argument is the same;
+    // otherwise capture it again.  The scratch arena is sized before the
+    // capture (no allocation or synchronisation may happen inside it).
+    a.flags = (flags | MH_CLASSIFY_BATCHED) & ~(MH_CLASSIFY_TIMING | MH_CLASSIFY_GRAPH);
+    const uint64_t key[12] = {(uint64_t)idx, (uint64_t)d_reads, (uint64_t)d_read_off, n_reads,
+                              a.flags, (uint64_t)d_out, (uint64_t)d_top_pool, top_cap,
+                              (uint64_t)d_cands, cand_cap, tiles_version(h), x.tie_factor};
+    if (!x.graph || std::memcmp(key, x.graph_key, sizeof(key)) != 0) {
+      if (x.graph) {
+        hipGraphExecDestroy(x.graph);
+        x.graph = nullptr;
+      }
+      st = ensure_scratch(h, x, classify_scratch_bytes_f(n_reads, h->max_read_len,
+                                                         idx->threshold / 2, x.tie_factor));
+      if (st) return st;
+      hipGraph_t g = nullptr;
+      MH_HIP(h, hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));
+      x.capturing = true;
+      st = launch_classify(h, x, a, true);
+      x.capturing = false;
+      hipError_t e = hipStreamEndCapture(x.stream, &g);
+      if (st) {
+        if (g) hipGraphDestroy(g);
+        return st;
+      }
+      if (e != hipSuccess) return hip_error(h, e, "graph capture");
+      e = hipGraphInstantiate(&x.graph, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        x.graph = nullptr;
+        return hip_error(h, e, "graph instantiate");
+      }
+      std::memcpy(x.graph_key, key, sizeof(key));
+    }
+    MH_HIP(h, hipGraphLaunch(x.graph, x.stream));
+    st = MH_OK;
+  } else {
+    st = launch_classify(h, x, a, (flags & MH_CLASSIFY_BATCHED) != 0);
+  }
   if (st) return st;
   if (!wait) return MH_OK;
   MH_HIP(h, hipMemcpyAsync(x.h_ctr, x.d_ctr, sizeof(DevCounters), hipMemcpyDeviceToHost,
@@ -406,6 +450,7 @@ mh_status mh_set_tiles(mh_handle *h, const mh_tile *tiles, uint32_t n_tiles) {
   if (!h || !tiles || !valid_tiles(tiles, n_tiles, h->max_read_len)) return MH_ERR_ARG;
   std::lock_guard<std::mutex> lk(h->mu);
   h->tiles.assign(tiles, tiles + n_tiles);
+  h->tiles_version++;
   return MH_OK;
 }
 

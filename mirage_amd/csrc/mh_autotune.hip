@@ -42,6 +42,7 @@ extern "C" mh_status mh_autotune(mh_handle *h, const mh_index *idx, const uint8_
     for (uint32_t it = 0; it <= iters; it++) {
       std::lock_guard<std::mutex> lk(h->mu);
       h->tiles = tt;
+      h->tiles_version++;
       ClassifyArgs a;
       a.ix = idx->view();
       a.luts = h->luts();
@@ -86,6 +87,7 @@ extern "C" mh_status mh_autotune(mh_handle *h, const mh_index *idx, const uint8_
   {
     std::lock_guard<std::mutex> lk(h->mu);
     h->tiles = tiles;
+    h->tiles_version++;
   }
   hipFree(d_out);
   hipFree(d_top);

@@ -1009,16 +1009,19 @@ __global__ void k_zero(unsigned long long *p, uint32_t n) {
 
 // Length buckets for the batched form: bucket of each read by its length.
 // One read per thread; counts are gathered in LDS and added once per block.
-__device__ __forceinline__ uint32_t bucket_of(const Reads &rd, uint32_t r, const uint32_t *max_len,
+struct Lens {
+  uint32_t v[8];
+};
+
+__device__ __forceinline__ uint32_t bucket_of(const Reads &rd, uint32_t r, const Lens &max_len,
                                               uint32_t nb) {
   const uint64_t m = rd.off[r + 1] - rd.off[r];
   uint32_t b = 0;
-  while (b + 1 < nb && m > max_len[b]) b++;
+  while (b + 1 < nb && m > max_len.v[b]) b++;
   return b;
 }
 
-__global__ void k_bin_count(Reads rd, const uint32_t *max_len, uint32_t nb,
-                            unsigned long long *cnt) {
+__global__ void k_bin_count(Reads rd, Lens max_len, uint32_t nb, unsigned long long *cnt) {
   __shared__ unsigned int loc[8];
   if (threadIdx.x < 8) loc[threadIdx.x] = 0;
   __syncthreads();
@@ -1028,7 +1031,7 @@ __global__ void k_bin_count(Reads rd, const uint32_t *max_len, uint32_t nb,
   if (threadIdx.x < nb && loc[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], (unsigned long long)loc[threadIdx.x]);
 }
 
-__global__ void k_bin_scatter(Reads rd, const uint32_t *max_len, uint32_t nb,
+__global__ void k_bin_scatter(Reads rd, Lens max_len, uint32_t nb,
                               const unsigned long long *start, unsigned long long *cursor,
                               uint32_t *list) {
   __shared__ unsigned int loc[8];
@@ -1124,6 +1127,8 @@ __global__ __launch_bounds__(64) void k_score_seeds(IndexView ix, Luts lut, cons
 
 bool g_comp_ready[64] = {};
 
+}  // namespace
+
 hipError_t ensure_comp_table(int device) {
   if (device >= 0 && device < 64 && g_comp_ready[device]) return hipSuccess;
   uint8_t t[256] = {};
@@ -1134,6 +1139,8 @@ hipError_t ensure_comp_table(int device) {
   if (e == hipSuccess && device >= 0 && device < 64) g_comp_ready[device] = true;
   return e;
 }
+
+namespace {
 
 template <int BS>
 hipError_t launch_lds(const Ctx &c, const Caps &cap, const uint32_t *qlist,
@@ -1258,9 +1265,13 @@ size_t classify_scratch_bytes(uint32_t n_reads, uint32_t max_read_len, int s) {
   return layout(n_reads, max_read_len, s, 4).total;
 }
 
+size_t classify_scratch_bytes_f(uint32_t n_reads, uint32_t max_read_len, int s,
+                                uint32_t tie_factor) {
+  return layout(n_reads, max_read_len, s, tie_factor).total;
+}
+
 mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool batched) {
-  hipError_t e = ensure_comp_table(h->device);
-  if (e != hipSuccess) return hip_error(h, e, "complement table");
+  hipError_t e = hipSuccess;
   const uint32_t n = a.rd.n;
   const uint64_t nq = 2ull * n;
   const Layout L = layout(n, h->max_read_len, a.ix.s, x.tie_factor);
@@ -1293,7 +1304,7 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
   c.out.tie_cap = L.tie_cap;
   hipStream_t s = x.stream;
   const uint32_t zn = sizeof(DevCounters) / 8;
-  const bool timed = (a.flags & MH_CLASSIFY_TIMING) != 0;
+  const bool timed = (a.flags & MH_CLASSIFY_TIMING) != 0 && !x.capturing;
   x.timed = false;
   x.timed_buckets = 0;
   if (timed) hipEventRecord(x.ev[0], s);
@@ -1326,11 +1337,8 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
       }
     } else {
       const uint32_t nb = (uint32_t)std::min<size_t>(tiles.size(), 8);
-      uint32_t *d_maxlen = (uint32_t *)(bw + 32);
-      uint32_t ml[8];
-      for (uint32_t i = 0; i < nb; i++) ml[i] = tiles[i].max_len;
-      e = hipMemcpyAsync(d_maxlen, ml, nb * 4, hipMemcpyHostToDevice, s);
-      if (e != hipSuccess) return hip_error(h, e, "tile lengths");
+      Lens d_maxlen{};
+      for (uint32_t i = 0; i < nb; i++) d_maxlen.v[i] = tiles[i].max_len;
       const Reads rd = a.rd;
       hipLaunchKernelGGL(k_bin_count, dim3(gr), dim3(256), 0, s, rd, d_maxlen, nb, bw);
       hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, bw, nb, bw + 16);

@@ -140,6 +140,10 @@ struct Exec {
   // bucket kernels, [10] after the large kernel, [11] after the merge
   hipEvent_t ev[12] = {};
   uint32_t tie_factor = 4;       // tie pool entries per query, grown on overflow
+  // captured batched call (MH_CLASSIFY_GRAPH) and the arguments it was made for
+  bool capturing = false;
+  hipGraphExec_t graph = nullptr;
+  uint64_t graph_key[12] = {};
   bool timed = false;
   uint32_t timed_buckets = 0;
   uint32_t bucket_order[8] = {};
@@ -162,6 +166,7 @@ struct mh_handle {
   mh_counters last{};
   bool last_valid = false;
   std::vector<mh_tile> tiles;
+  uint64_t tiles_version = 0;
   mh::Luts luts() const { return mh::Luts{d_x_log, d_smax_log, n_x, n_smax}; }
 };
 
@@ -196,8 +201,12 @@ struct ClassifyArgs {
   mh_read_result *final_out;
 };
 mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool batched);
+// Complement table in constant memory (once per device, before any capture).
+hipError_t ensure_comp_table(int device);
 // Scratch bytes one classify call needs.
 size_t classify_scratch_bytes(uint32_t n_reads, uint32_t max_read_len, int s);
+size_t classify_scratch_bytes_f(uint32_t n_reads, uint32_t max_read_len, int s,
+                                uint32_t tie_factor);
 // Reads the counters of the last call on x (x's stream must be idle).
 void counters_from(const DevCounters &d, uint64_t n_reads, mh_counters &out);
 mh_status launch_score_seeds(mh_handle *h, const IndexView &ix, const Luts &luts,

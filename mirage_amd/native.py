@@ -17,6 +17,7 @@ MH_INDEX_LAST_BUCKET_EMPTY = 1
 MH_CLASSIFY_ULTRAFAST = 1
 MH_CLASSIFY_NO_DIRECT = 2
 MH_CLASSIFY_BATCHED = 4
+MH_CLASSIFY_GRAPH = 16
 
 READ_RESULT_DTYPE = np.dtype([("highest", "<f4"), ("strand", "<u4"), ("n_top", "<u4"),
                               ("top_off", "<u4"), ("n_cand", "<u4"), ("status", "<u4")])
