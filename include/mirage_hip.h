@@ -107,7 +107,9 @@ typedef struct {
   uint64_t seeds;          /* seeds scored */
   uint64_t cands;          /* seeded candidates */
   uint64_t direct;         /* strands resolved by the direct-hit fast path */
-  uint64_t overflow;       /* strands sent to the large workspace */
+  uint64_t overflow;       /* tier escalations: strands passed from their length tile
+                              to the middle tier, from it to the HBM tiers, and from
+                              the first HBM tier to the last resort */
   uint64_t ext_compares;   /* base comparisons of head/tail extension */
 } mh_counters;
 
@@ -115,7 +117,7 @@ typedef struct {
 typedef struct {
   float total_ms;          /* first to last kernel of the call */
   float bucket_ms[8];      /* LDS kernel per length bucket (plain form: [0]) */
-  float big_ms;            /* large-workspace kernel */
+  float big_ms;            /* middle tier and HBM-workspace tiers */
   float merge_ms;          /* strand merge */
   uint32_t n_buckets;
 } mh_timing;

@@ -200,7 +200,7 @@ struct Ws {
   uint32_t *v1;        // ecap
   uint64_t *k2;        // ecap
   uint32_t *v2;        // ecap
-  uint32_t *t1, *t2, *t3, *rref;  // ecap each
+  uint32_t *t1, *t2, *t3, *rref;  // ecap each, one contiguous region (4 * ecap)
   uint32_t *bitmap;    // qcap / 32 + 1
   uint32_t *red;       // 64 words
   unsigned long long *acc;  // 2
@@ -214,7 +214,7 @@ __host__ __device__ inline size_t ws_bytes(const Caps &c) {
   b += al8(c.qcap);
   b += al8(c.pcap * 4ull) + al8(c.pcap * 8ull) + al8((c.pcap + 1) * 4ull);
   b += 2 * (al8(c.ecap * 8ull) + al8(c.ecap * 4ull));
-  b += 4 * al8(c.ecap * 4ull);
+  b += al8(c.ecap * 16ull);
   b += al8((c.qcap / 32 + 1) * 4ull);
   b += 64 * 4 + 16 + 64;
   return b;
@@ -236,10 +236,10 @@ __device__ inline Ws ws_carve(unsigned char *base, const Caps &c) {
   w.v1 = (uint32_t *)take(c.ecap * 4ull);
   w.k2 = (uint64_t *)take(c.ecap * 8ull);
   w.v2 = (uint32_t *)take(c.ecap * 4ull);
-  w.t1 = (uint32_t *)take(c.ecap * 4ull);
-  w.t2 = (uint32_t *)take(c.ecap * 4ull);
-  w.t3 = (uint32_t *)take(c.ecap * 4ull);
-  w.rref = (uint32_t *)take(c.ecap * 4ull);
+  w.t1 = (uint32_t *)take(c.ecap * 16ull);
+  w.t2 = w.t1 + c.ecap;
+  w.t3 = w.t1 + 2ull * c.ecap;
+  w.rref = w.t1 + 3ull * c.ecap;
   w.bitmap = (uint32_t *)take((c.qcap / 32 + 1) * 4ull);
   w.red = (uint32_t *)take(64 * 4);
   w.acc = (unsigned long long *)take(16);
@@ -252,6 +252,7 @@ struct Ctx {
   Luts lut;
   Reads rd;
   uint32_t flags;
+  uint32_t max_len;  // longest read the handle accepts
   Outputs out;
 };
 
@@ -393,6 +394,202 @@ __device__ __forceinline__ void write_query(const Ctx &c, uint32_t q, const Quer
   if (threadIdx.x == 0) c.out.q[q] = r;
 }
 
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t hash_slot(uint32_t key, uint32_t mask) {
+  return (key * 0x9E3779B1u) & mask;
+}
+
+// Inserts key into an open-addressing set; returns true when it was new.
+__device__ __forceinline__ bool set_insert(uint32_t *set, uint32_t mask, uint32_t key) {
+  for (uint32_t s = hash_slot(key, mask);; s = (s + 1) & mask) {
+    const uint32_t old = atomicCAS(&set[s], kEmpty, key);
+    if (old == kEmpty) return true;
+    if (old == key) return false;
+  }
+}
+
+// Slot of key in an open-addressing map, inserting it when absent.
+__device__ __forceinline__ uint32_t map_slot(uint32_t *keys, uint32_t mask, uint32_t key) {
+  for (uint32_t s = hash_slot(key, mask);; s = (s + 1) & mask) {
+    const uint32_t old = atomicCAS(&keys[s], kEmpty, key);
+    if (old == kEmpty || old == key) return s;
+  }
+}
+
+// Runs of consecutive hit indices over sorted pair keys: p = key >> KS
+// holds the hit index in its low RS bits and the reference above them.
+// Returns, per element, the length of its run in `ids` (distinct hits; a
+// repeat of one hit in a bucket does not lengthen a run, Analyser.cpp:2057-
+// 2058), using `len` as scratch.  All threads get the maximum.
+template <int BS, int KS, int RS>
+__device__ __forceinline__ uint32_t run_flags(const uint64_t *key, uint32_t t) {
+  if (t == 0) return 3u;
+  const uint64_t p = key[t] >> KS, pp = key[t - 1] >> KS;
+  const bool same = (p >> RS) == (pp >> RS);
+  return ((!same || (p != pp && p != pp + 1)) ? 1u : 0u) | ((!same || p != pp) ? 2u : 0u);
+}
+
+template <int BS, int KS, int RS>
+__device__ uint32_t run_lengths(const Ws &ws, const uint64_t *key, uint32_t n, uint32_t *ids,
+                                uint32_t *len) {
+  const uint32_t tid = threadIdx.x;
+  uint32_t nruns = 0;
+  for (uint32_t base = 0; base < n; base += BS) {
+    const uint32_t t = base + tid;
+    const uint32_t st = t < n ? (run_flags<BS, KS, RS>(key, t) & 1u) : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<BS>(st, ws.red, tot);
+    if (t < n) ids[t] = nruns + ex + st - 1;
+    nruns += tot;
+  }
+  __syncthreads();
+  for (uint32_t r = tid; r < nruns; r += BS) len[r] = 0;
+  __syncthreads();
+  for (uint32_t t = tid; t < n; t += BS)
+    if (run_flags<BS, KS, RS>(key, t) & 2u) atomicAdd(&len[ids[t]], 1u);
+  __syncthreads();
+  uint32_t mx = 0;
+  for (uint32_t t = tid; t < n; t += BS) {
+    const uint32_t L = len[ids[t]];
+    mx = max(mx, L);
+    ids[t] = L;  // per element from here on
+  }
+  const uint32_t m = block_max<BS>(mx, ws.red);
+  __syncthreads();
+  return m;
+}
+
+// Successive-occurrence filter with hashing.  A reference with a single
+// bucket entry has a run of 1; only references with two or more entries
+// (the read's own reference and repeats, few) are sorted by (ref, hit) to
+// measure their runs.  The reference map has its values in t1..t2 and its
+// keys in t3..rref (2E slots each); the keys are free again after the
+// inserts and hold the run scratch.
+template <int BS>
+__device__ void filter_hashed(const Ws &ws, uint32_t E, uint32_t &nP) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t cap = next_pow2(2 * E), mask = cap - 1;
+  uint32_t *rval = ws.t1, *rkeys = ws.t3;
+  for (uint32_t i = tid; i < cap; i += BS) {
+    rkeys[i] = kEmpty;
+    rval[i] = 0;
+  }
+  __syncthreads();
+  for (uint32_t e = tid; e < E; e += BS) {
+    const uint32_t r = (uint32_t)(ws.k1[e] >> 32);
+    const uint32_t slot = map_slot(rkeys, mask, r);
+    atomicAdd(&rval[slot], 1u);
+    ws.v2[e] = slot;  // map slot of each pair
+  }
+  __syncthreads();
+  // Pairs of references with two or more entries -> k2 as
+  // ((ref << 12 | hit) << 32 | slot).
+  uint32_t nm = 0;
+  for (uint32_t base = 0; base < E; base += BS) {
+    const uint32_t e = base + tid;
+    uint32_t multi = 0, slot = 0;
+    uint64_t key = 0;
+    if (e < E) {
+      slot = ws.v2[e];
+      key = ws.k1[e];
+      multi = rval[slot] >= 2 ? 1u : 0u;
+    }
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<BS>(multi, ws.red, tot);
+    if (multi)
+      ws.k2[nm + ex] =
+          ((uint64_t)(((uint32_t)(key >> 32) << 12) | (uint32_t)key) << 32) | slot;
+    nm += tot;
+  }
+  __syncthreads();
+  // Every reference starts with a run of 1; the sorted pass raises it.
+  for (uint32_t i = tid; i < cap; i += BS) rval[i] = 1;
+  const uint32_t n2 = next_pow2(nm);
+  for (uint32_t e = nm + tid; e < n2; e += BS) ws.k2[e] = ~0ull;
+  __syncthreads();
+  uint32_t maxrun = 0;
+  if (nm) {
+    bitonic_sort<BS>(ws.k2, ws.t3, n2);  // t3: value scratch, unused
+    maxrun = run_lengths<BS, 32, 12>(ws, ws.k2, nm, ws.t3, ws.rref);
+    for (uint32_t t = tid; t < nm; t += BS) atomicMax(&rval[(uint32_t)ws.k2[t]], ws.t3[t]);
+    __syncthreads();
+  }
+  const uint32_t cut = maxrun >= 2 ? maxrun - 1 : 1;
+  // Kept pairs, compacted in place into k2/v2 (position <= source index;
+  // each chunk is read before it is written).
+  uint32_t nkeep = 0;
+  for (uint32_t base = 0; base < E; base += BS) {
+    const uint32_t e = base + tid;
+    uint32_t keep = 0, pos = 0, j = 0;
+    uint64_t key = 0;
+    if (e < E) {
+      key = ws.k1[e];
+      pos = ws.v1[e];
+      j = ws.hit_j[(uint32_t)key];
+      keep = rval[ws.v2[e]] >= cut ? 1u : 0u;
+    }
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<BS>(keep, ws.red, tot);
+    __syncthreads();
+    if (keep) {
+      ws.k2[nkeep + ex] = (key & 0xFFFFFFFF00000000ull) | pos;
+      ws.v2[nkeep + ex] = j;
+    }
+    nkeep += tot;
+  }
+  nP = nkeep;
+  __syncthreads();
+}
+
+// Successive-occurrence filter by sorting every pair by (ref, hit); used
+// when (ref, hit) does not pack into 32 bits.
+template <int BS>
+__device__ void filter_sorted(const Ws &ws, uint32_t E, uint32_t &nP) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t n2 = next_pow2(E);
+  for (uint32_t e = E + tid; e < n2; e += BS) {
+    ws.k1[e] = ~0ull;
+    ws.v1[e] = ~0u;
+  }
+  bitonic_sort<BS>(ws.k1, ws.v1, n2);
+  const uint32_t maxrun = run_lengths<BS, 0, 32>(ws, ws.k1, E, ws.t2, ws.t3);
+  const uint32_t cut = maxrun >= 2 ? maxrun - 1 : 1;
+  // longest run per reference segment
+  for (uint32_t t = tid; t < E; t += BS)
+    ws.t1[t] = (t == 0 || (ws.k1[t - 1] >> 32) != (ws.k1[t] >> 32)) ? 1u : 0u;
+  __syncthreads();
+  uint32_t nseg = 0;
+  for (uint32_t base = 0; base < E; base += BS) {
+    const uint32_t t = base + tid;
+    const uint32_t st = t < E ? ws.t1[t] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<BS>(st, ws.red, tot);
+    if (t < E) ws.t3[t] = nseg + ex + st - 1;
+    nseg += tot;
+  }
+  __syncthreads();
+  for (uint32_t r = tid; r < nseg; r += BS) ws.rref[r] = 0;
+  __syncthreads();
+  for (uint32_t t = tid; t < E; t += BS) atomicMax(&ws.rref[ws.t3[t]], ws.t2[t]);
+  __syncthreads();
+  uint32_t nkeep = 0;
+  for (uint32_t base = 0; base < E; base += BS) {
+    const uint32_t t = base + tid;
+    const uint32_t keep = (t < E && ws.rref[ws.t3[t]] >= cut) ? 1u : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan<BS>(keep, ws.red, tot);
+    if (keep) {
+      const uint64_t key = ws.k1[t];
+      ws.k2[nkeep + ex] = (key & 0xFFFFFFFF00000000ull) | ws.v1[t];
+      ws.v2[nkeep + ex] = ws.hit_j[(uint32_t)key];
+    }
+    nkeep += tot;
+  }
+  nP = nkeep;
+  __syncthreads();
+}
+
 // Processes query q.  Returns kOverflow (uniformly) when the working set
 // exceeds the workspace capacities; nothing is written in that case.
 template <int BS>
@@ -414,6 +611,11 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
   res.n_cand = 0;
   res.status = 0;
   res.pad = 0;
+  if (m64 > c.max_len) {
+    res.status = MH_READ_TOO_LONG;
+    write_query(c, q, res);
+    return kDone;
+  }
   if (m64 > cap.qcap) {
     if (!last_resort) return kOverflow;
     res.status = MH_READ_TOO_LONG;
@@ -427,9 +629,9 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
   int bad = 0;
   for (uint32_t x = tid; x < m; x += BS) {
     uint8_t ch = upper(c.rd.bytes[r0 + (strand ? (m - 1 - x) : x)]);
-    if (strand) ch = kComp[ch];
-    bad |= (ch == 0);
-    ws.q[x] = ch;
+    const uint8_t cc = kComp[ch];  // zero outside the nucleotide alphabet
+    bad |= (cc == 0);
+    ws.q[x] = strand ? cc : ch;
   }
   bad = __syncthreads_or(bad);
   if (bad) {
@@ -615,95 +817,15 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
   }
 
   if (!direct) {
-    // Successive-occurrence filter.  Sort by (ref, hit) and measure runs of
-    // consecutive hit indices; repeats of one hit inside a bucket do not
-    // lengthen a run (Analyser.cpp:2057-2058).
-    const uint32_t n2 = next_pow2(E);
-    for (uint32_t e = E + tid; e < n2; e += BS) {
-      ws.k1[e] = ~0ull;
-      ws.v1[e] = ~0u;
-    }
-    bitonic_sort<BS>(ws.k1, ws.v1, n2);
-    // t1 flags: bit0 run start, bit1 new hit index, bit2 reference start
-    for (uint32_t t = tid; t < E; t += BS) {
-      const uint64_t key = ws.k1[t];
-      uint32_t f = 7;
-      if (t > 0) {
-        const uint64_t pk = ws.k1[t - 1];
-        const bool same = (pk >> 32) == (key >> 32);
-        const uint32_t hit = (uint32_t)key, phit = (uint32_t)pk;
-        f = ((!same || (hit != phit && hit != phit + 1)) ? 1u : 0u) |
-            ((!same || hit != phit) ? 2u : 0u) | (!same ? 4u : 0u);
-      }
-      ws.t1[t] = f;
-    }
-    __syncthreads();
-    uint32_t nruns = 0;
-    for (uint32_t base = 0; base < E; base += BS) {
-      const uint32_t t = base + tid;
-      const uint32_t st = t < E ? (ws.t1[t] & 1u) : 0u;
-      uint32_t tot;
-      const uint32_t ex = block_excl_scan<BS>(st, ws.red, tot);
-      if (t < E) ws.t2[t] = nruns + ex + st - 1;  // run id
-      nruns += tot;
-    }
-    __syncthreads();
-    for (uint32_t r = tid; r < nruns; r += BS) ws.t3[r] = 0;
-    __syncthreads();
-    for (uint32_t t = tid; t < E; t += BS)
-      if (ws.t1[t] & 2u) atomicAdd(&ws.t3[ws.t2[t]], 1u);
-    __syncthreads();
-    uint32_t mx = 0;
-    for (uint32_t t = tid; t < E; t += BS) {
-      const uint32_t L = ws.t3[ws.t2[t]];
-      ws.t2[t] = L;  // length of the element's run
-      mx = max(mx, L);
-    }
-    const uint32_t maxrun = block_max<BS>(mx, ws.red);
-    // highestSuccesive moves only on a continued run (Analyser.cpp:2067-2072),
-    // and the cut is max(1, highest - 1) (:2099).
-    const uint32_t cut = maxrun >= 2 ? maxrun - 1 : 1;
-    __syncthreads();
-    uint32_t nrefseg = 0;
-    for (uint32_t base = 0; base < E; base += BS) {
-      const uint32_t t = base + tid;
-      const uint32_t st = t < E ? ((ws.t1[t] >> 2) & 1u) : 0u;
-      uint32_t tot;
-      const uint32_t ex = block_excl_scan<BS>(st, ws.red, tot);
-      if (t < E) ws.t3[t] = nrefseg + ex + st - 1;  // reference segment id
-      nrefseg += tot;
-    }
-    __syncthreads();
-    for (uint32_t r = tid; r < nrefseg; r += BS) ws.rref[r] = 0;
-    __syncthreads();
-    for (uint32_t t = tid; t < E; t += BS) atomicMax(&ws.rref[ws.t3[t]], ws.t2[t]);
-    __syncthreads();
-    for (uint32_t t = tid; t < E; t += BS) ws.t2[t] = ws.rref[ws.t3[t]] >= cut ? 1u : 0u;
-    __syncthreads();
-    // Kept pairs, ranked by reference: k2 = rank << 32 | refPos, v2 = qPos.
-    uint32_t nkeep = 0, nrank = 0;
-    for (uint32_t base = 0; base < E; base += BS) {
-      const uint32_t t = base + tid;
-      uint32_t keep = 0, rs = 0;
-      if (t < E) {
-        keep = ws.t2[t];
-        rs = keep & ((ws.t1[t] >> 2) & 1u);
-      }
-      uint32_t tk, tr;
-      const uint32_t ek = block_excl_scan<BS>(keep, ws.red, tk);
-      const uint32_t er = block_excl_scan<BS>(rs, ws.red, tr);
-      if (keep) {
-        const uint64_t key = ws.k1[t];
-        const uint32_t rank = nrank + er + rs - 1;
-        ws.k2[nkeep + ek] = ((uint64_t)rank << 32) | ws.v1[t];
-        ws.v2[nkeep + ek] = ws.hit_j[(uint32_t)key];
-        if (rs) ws.rref[rank] = (uint32_t)(key >> 32);
-      }
-      nkeep += tk;
-      nrank += tr;
-    }
-    nP = nkeep;
-    nranks = nrank;
+    // Successive-occurrence filter (Analyser.cpp:2004-2197): the longest run
+    // of consecutive hit indices of each reference, M the longest of all
+    // (counted only from 2, :2067-2072), cut = max(1, M - 1) (:2099); every
+    // pair of a reference whose run reaches the cut is kept.  Result: kept
+    // pairs in k2/v2 as (ref << 32 | refPos, qPos), unsorted.
+    if (nhits < (1u << 12) && ix.n_refs < (1u << 20))
+      filter_hashed<BS>(ws, E, nP);
+    else
+      filter_sorted<BS>(ws, E, nP);
     // Sort by (reference, refPos, qPos) (Analyser.cpp:1024-1059).
     const uint32_t p2 = next_pow2(nP);
     __syncthreads();
@@ -712,6 +834,27 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
       ws.v2[e] = ~0u;
     }
     bitonic_sort<BS>(ws.k2, ws.v2, p2);
+    // Candidate references in ascending order: rank = index among the
+    // distinct references; k2 becomes rank << 32 | refPos.
+    for (uint32_t t = tid; t < nP; t += BS)
+      ws.t1[t] = (t == 0 || (ws.k2[t - 1] >> 32) != (ws.k2[t] >> 32)) ? 1u : 0u;
+    __syncthreads();
+    uint32_t nrank = 0;
+    for (uint32_t base = 0; base < nP; base += BS) {
+      const uint32_t t = base + tid;
+      const uint32_t st = t < nP ? ws.t1[t] : 0u;
+      uint32_t tot;
+      const uint32_t ex = block_excl_scan<BS>(st, ws.red, tot);
+      if (t < nP) {
+        const uint64_t key = ws.k2[t];
+        const uint32_t rank = nrank + ex + st - 1;
+        if (st) ws.rref[rank] = (uint32_t)(key >> 32);
+        ws.k2[t] = ((uint64_t)rank << 32) | (uint32_t)key;
+      }
+      nrank += tot;
+    }
+    nranks = nrank;
+    __syncthreads();
     // Chains: neighbours link when both coordinates step by s
     // (Analyser.cpp:2351-2353, u32 arithmetic).
     for (uint32_t t = tid; t < nP; t += BS) {
@@ -932,10 +1075,15 @@ __global__ __launch_bounds__(BS) void k_classify(Ctx c, Caps cap, const uint32_t
   stats_flush(ws, c.out.ctr);
 }
 
-// Queries that overflowed both LDS tiers, with the workspace in HBM.
-__global__ __launch_bounds__(1024) void k_classify_big(Ctx c, Caps cap, unsigned char *scratch,
-                                                       size_t ws_stride, const uint32_t *list,
-                                                       const unsigned long long *count) {
+// Queries that overflowed both LDS tiers, with the workspace in HBM: one
+// workspace per workgroup, queries taken one at a time from `list`.  The
+// last resort flags what it cannot hold; the tier before it passes such
+// queries on through c.out.overflow.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_classify_hbm(Ctx c, Caps cap, unsigned char *scratch,
+                                                     size_t ws_stride, const uint32_t *list,
+                                                     const unsigned long long *count,
+                                                     unsigned long long *head, int last_resort) {
   const Ws ws = ws_carve(scratch + blockIdx.x * ws_stride, cap);
   __shared__ uint32_t slot;
   stats_begin(ws);
@@ -943,12 +1091,16 @@ __global__ __launch_bounds__(1024) void k_classify_big(Ctx c, Caps cap, unsigned
     __syncthreads();
     if (threadIdx.x == 0) {
       const unsigned long long n = *count;
-      const unsigned long long i = atomicAdd(&c.out.ctr->big_overflow_n, 1ull);
+      const unsigned long long i = atomicAdd(head, 1ull);
       slot = (i < n && i < c.out.overflow_cap) ? (uint32_t)i : 0xFFFFFFFFu;
     }
     __syncthreads();
     if (slot == 0xFFFFFFFFu) break;
-    process_query<1024>(c, ws, cap, list[slot], true);
+    const uint32_t q = list[slot];
+    if (process_query<BS>(c, ws, cap, q, last_resort != 0) == kOverflow && threadIdx.x == 0) {
+      const unsigned long long o = atomicAdd(c.out.overflow_cnt, 1ull);
+      if (o < c.out.overflow_cap) c.out.overflow[o] = q;
+    }
   }
   stats_flush(ws, c.out.ctr);
 }
@@ -1207,17 +1359,38 @@ Caps caps_for(const mh_tile &t, int s, size_t limit) {
   return c;
 }
 
+// HBM tiers: a first one with moderate workspaces over many workgroups,
+// then the last resort with the largest entry capacity over a few.  They run
+// one after the other and share one region.
+constexpr uint32_t kHbmBlocks = 512;
+constexpr uint32_t kHbmEcap = 1u << 15;
 constexpr uint32_t kBigBlocks = 32;
 constexpr uint32_t kBigEcap = 1u << 19;
+
+Caps hbm_caps(uint32_t max_read_len, int s, uint32_t ecap) {
+  Caps c;
+  c.qcap = (max_read_len + 63) & ~63u;
+  c.pcap = c.qcap / (uint32_t)(s > 0 ? s : 1) + 2;
+  c.ecap = ecap;
+  return c;
+}
+
+size_t ws_stride(const Caps &c) { return (ws_bytes(c) + 255) & ~size_t(255); }
 
 }  // namespace
 
 std::vector<mh_tile> default_tiles(uint32_t max_read_len) {
+  const mh_tile base[4] = {{320, 64, 64, 0}, {1024, 128, 512, 0}, {4096, 256, 2048, 0},
+                           {0xFFFFFFFFu, 256, 2048, 0}};
   std::vector<mh_tile> t;
-  t.push_back(mh_tile{320, 64, 64, 0});
-  t.push_back(mh_tile{1024, 128, 512, 0});
-  t.push_back(mh_tile{4096, 256, 2048, 0});
-  t.push_back(mh_tile{std::max<uint32_t>(max_read_len, 4097), 256, 2048, 0});
+  for (mh_tile b : base) {
+    if (b.max_len >= max_read_len) {
+      b.max_len = max_read_len;
+      t.push_back(b);
+      break;
+    }
+    t.push_back(b);
+  }
   return t;
 }
 
@@ -1252,11 +1425,8 @@ Layout layout(uint32_t n_reads, uint32_t max_read_len, int s, uint32_t tie_facto
   hipcub::DeviceScan::ExclusiveSum(nullptr, L.scan_bytes, (const uint32_t *)nullptr,
                                    (uint32_t *)nullptr, (int)(n_reads + 1));
   L.scan = take(L.scan_bytes);
-  Caps big;
-  big.qcap = (max_read_len + 63) & ~63u;
-  big.pcap = big.qcap / (uint32_t)(s > 0 ? s : 1) + 2;
-  big.ecap = kBigEcap;
-  L.big = take(kBigBlocks * ((ws_bytes(big) + 255) & ~size_t(255)));
+  L.big = take(std::max(kHbmBlocks * ws_stride(hbm_caps(max_read_len, s, kHbmEcap)),
+                        kBigBlocks * ws_stride(hbm_caps(max_read_len, s, kBigEcap))));
   L.total = o;
   return L;
 }
@@ -1294,6 +1464,7 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
   c.lut = a.luts;
   c.rd = a.rd;
   c.flags = a.flags;
+  c.max_len = h->max_read_len;
   c.out = a.out;
   c.out.q = qo;
   c.out.ctr = x.d_ctr;
@@ -1375,13 +1546,22 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
                       &x.d_ctr->mid_head, mgrid, s);
       if (e != hipSuccess) return hip_error(h, e, "k_classify middle tier");
     }
-    Caps big;
-    big.qcap = (h->max_read_len + 63) & ~63u;
-    big.pcap = big.qcap / (uint32_t)(a.ix.s > 0 ? a.ix.s : 1) + 2;
-    big.ecap = kBigEcap;
-    const size_t stride = (ws_bytes(big) + 255) & ~size_t(255);
-    hipLaunchKernelGGL(k_classify_big, dim3(kBigBlocks), dim3(1024), 0, s, c, big, big_ws, stride,
-                       (const uint32_t *)overflow2, (const unsigned long long *)&x.d_ctr->overflow2_n);
+    {
+      // First HBM tier, then the last resort; the tile overflow list is free
+      // again and carries what the first tier passes on.
+      const Caps ca = hbm_caps(h->max_read_len, a.ix.s, kHbmEcap);
+      Ctx ch = c;
+      ch.out.overflow = overflow;
+      ch.out.overflow_cnt = &x.d_ctr->overflow3_n;
+      hipLaunchKernelGGL(k_classify_hbm<256>, dim3(kHbmBlocks), dim3(256), 0, s, ch, ca, big_ws,
+                         ws_stride(ca), (const uint32_t *)overflow2,
+                         (const unsigned long long *)&x.d_ctr->overflow2_n,
+                         &x.d_ctr->queue_head[8], 0);
+      const Caps cb = hbm_caps(h->max_read_len, a.ix.s, kBigEcap);
+      hipLaunchKernelGGL(k_classify_hbm<1024>, dim3(kBigBlocks), dim3(1024), 0, s, c, cb, big_ws,
+                         ws_stride(cb), (const uint32_t *)overflow,
+                         (const unsigned long long *)&x.d_ctr->overflow3_n, &x.d_ctr->big_head, 1);
+    }
     if (timed) hipEventRecord(x.ev[10], s);
     hipLaunchKernelGGL(k_merge_count, dim3(std::min<uint32_t>(gr, 4096)), dim3(256), 0, s,
                        (const QueryOut *)qo, n, a.final_out, mcnt);
@@ -1415,7 +1595,7 @@ void counters_from(const DevCounters &d, uint64_t n_reads, mh_counters &o) {
   o.seeds = t[kSeeds];
   o.cands = t[kCands];
   o.direct = t[kDirect];
-  o.overflow = d.overflow_n + d.overflow2_n;
+  o.overflow = d.overflow_n + d.overflow2_n + d.overflow3_n;
   o.ext_compares = t[kExt];
 }
 

@@ -66,12 +66,12 @@ constexpr int kStatSlots = 256;
 struct DevCounters {
   unsigned long long top_used;    // final top-tie pool fill (set by the merge)
   unsigned long long cand_used;
-  unsigned long long overflow_n;
-  unsigned long long big_overflow_n;
+  unsigned long long overflow_n;  // queries the length tiles pass to the middle tier
+  unsigned long long big_head;    // queue head of the last-resort HBM tier
   unsigned long long tie_used;    // scratch tie pool fill
-  unsigned long long overflow2_n; // queries the middle tier passes to the HBM workspace
+  unsigned long long overflow2_n; // queries the middle tier passes to the HBM tiers
   unsigned long long mid_head;    // middle-tier queue head
-  unsigned long long pad;
+  unsigned long long overflow3_n; // queries the first HBM tier passes to the last resort
   unsigned long long queue_head[16];
   unsigned long long stats[kStatSlots][8];
 };

@@ -1,0 +1,127 @@
+"""Overflow tiers and per-read statuses of the HIP path.
+
+A strand whose working set exceeds its length tile moves to a middle LDS
+tier, then to HBM workspaces (a moderate one over many workgroups, then the
+largest); the result must not depend on which tier ran.  Reads the path
+cannot classify are flagged in their status word instead of failing the
+batch.
+"""
+import numpy as np
+import pytest
+
+from mirage_amd import datagen as D
+from mirage_amd import native
+from oracle import oracle as O
+from tests.test_gpu_parity import compare_cands, compare_reads
+
+pytestmark = pytest.mark.gpu
+
+ACGT = np.frombuffer(b"ACGT", np.uint8)
+
+
+def shared_segment(n_refs, seg_len, flank, seed):
+    """n_refs references, each a shared segment between random flanks."""
+    rng = np.random.default_rng(seed)
+    seg = rng.choice(ACGT, seg_len)
+    parts, off = [], [0]
+    for _ in range(n_refs):
+        r = np.concatenate([rng.choice(ACGT, flank), seg, rng.choice(ACGT, flank)])
+        parts.append(r)
+        off.append(off[-1] + len(r))
+    return np.concatenate(parts), np.array(off, np.uint64), seg
+
+
+def pack(reads):
+    off = np.zeros(len(reads) + 1, np.uint64)
+    off[1:] = np.cumsum([len(r) for r in reads])
+    return np.concatenate(reads).astype(np.uint8), off
+
+
+def mutate(rng, r, n):
+    r = r.copy()
+    for p in rng.integers(0, len(r), n):
+        r[p] = ACGT[(np.searchsorted(ACGT, r[p]) + 1) % 4]
+    return r
+
+
+@pytest.mark.parametrize("flags", [0, native.MH_CLASSIFY_BATCHED])
+def test_repeat_reads_reach_every_tier(handle, flags):
+    """300 references share a 2.4 kb segment: reads from it gather 20k-47k
+    bucket entries per strand, beyond the LDS tiers and, for the longest,
+    beyond the first HBM tier (2^15 entries); 300-way ties fill the tie pool."""
+    rb, ro, seg = shared_segment(300, 2400, 500, seed=11)
+    rng = np.random.default_rng(12)
+    reads = []
+    for ln in (1000, 1200, 1500, 1800, 2000, 2400):
+        st = int(rng.integers(0, len(seg) - ln + 1))
+        reads.append(seg[st:st + ln])
+        reads.append(mutate(rng, seg[st:st + ln], 6))
+    for i in range(6):  # flank reads with a single reference
+        a = int(ro[i * 40]) + int(rng.integers(0, 300))
+        reads.append(rb[a:a + 150])
+    qb, qo = pack(reads)
+    ix = handle.build_index(rb, ro, 30)
+    out, top, cands = ix.classify_host(qb, qo, flags=flags, want_cands=True)
+    c = handle.counters()
+    assert c["entries"] > 3 * (1 << 15)
+    assert c["overflow"] >= 12
+    oix = O.Index(rb, ro, 30)
+    oout, otop, ocands = oix.classify(qb, qo, want_cands=True, top_cap=512)
+    compare_reads(out, top, oout, otop, "shared segment")
+    compare_cands(cands, ocands, "shared segment")
+    assert max(len(t) for t in native.tie_sets(out, top)) == 300
+
+
+@pytest.mark.parametrize("flags", [0, native.MH_CLASSIFY_BATCHED])
+def test_tiny_tiles_send_work_to_the_tiers(handle, flags):
+    """With 64-entry tiles most strands overflow; results are unchanged."""
+    saved = handle.get_tiles()
+    try:
+        handle.set_tiles([(320, 64, 64, 0), (handle.max_read_len, 64, 64, 0)])
+        w = D.workload(3, 0.0005)
+        w.refs.n = 300
+        rb, ro, qb, qo, _ = D.generate(w)
+        ix = handle.build_index(rb, ro, w.threshold)
+        out, top, cands = ix.classify_host(qb, qo, flags=flags, want_cands=True)
+        assert handle.counters()["overflow"] > 0
+    finally:
+        handle.set_tiles(saved)
+    oix = O.Index(rb, ro, w.threshold)
+    oout, otop, ocands = oix.classify(qb, qo, want_cands=True)
+    compare_reads(out, top, oout, otop, "tiny tiles")
+    compare_cands(cands, ocands, "tiny tiles")
+
+
+def test_read_statuses():
+    """Bad bases, reads over the handle's length limit and strands with more
+    entries than the largest workspace (2^19; here 1598 probes x 340
+    references) are flagged per read; the other reads of the batch are
+    classified as usual."""
+    h = native.Handle(0, 25000)
+    rb, ro, seg = shared_segment(340, 24000, 300, seed=21)
+    rng = np.random.default_rng(22)
+    good = [seg[100:400].copy(), rb[int(ro[5]) + 10:int(ro[5]) + 260].copy()]
+    bad = seg[500:800].copy()
+    bad[150] = ord("X")
+    lower = np.frombuffer(bytes(seg[1000:1300]).lower(), np.uint8).copy()
+    reads = [good[0], bad, rng.choice(ACGT, 25001), seg[:24000].copy(), good[1], lower]
+    qb, qo = pack(reads)
+    ix = h.build_index(rb, ro, 30)
+    for flags in (0, native.MH_CLASSIFY_BATCHED):
+        out, top, _ = ix.classify_host(qb, qo, flags=flags)
+        st = [int(x) for x in out["status"]]
+        assert st == [0, native.MH_READ_BADBASE, native.MH_READ_TOO_LONG,
+                      native.MH_READ_CAPACITY, 0, 0], st
+        for i in (1, 2):
+            assert out["n_top"][i] == 0 and out["n_cand"][i] == 0
+        keep = [0, 4, 5]
+        sb, so = pack([reads[i] for i in keep])
+        oout, otop = O.Index(rb, ro, 30).classify(sb, so, top_cap=512)[:2]
+        sub = out[keep]
+        ties = native.tie_sets(out, top)
+        oties = O.top_sets(oout, otop)
+        for j, i in enumerate(keep):
+            assert np.float32(sub["highest"][j]).view(np.uint32) == \
+                np.float32(oout["highest"][j]).view(np.uint32)
+            assert ties[i] == oties[j]
+    h.close()
