@@ -56,6 +56,8 @@ typedef struct mh_index mh_index;
 #define MH_CLASSIFY_TIMING 8u      /* record kernel timings (mh_get_timing) */
 #define MH_CLASSIFY_GRAPH 16u      /* batched form replayed from a captured hipGraph;
                                       captured again when any argument changes */
+#define MH_CLASSIFY_PREPROBE 32u   /* batched form: hash and look up the probes of the
+                                      shortest length tile in a separate pass */
 
 /* Per-read result after the strand merge (MirAge/Analyser.cpp:592-611).
  * Top-tie reference ids (ascending) are top_pool[top_off, top_off+n_top).
@@ -203,8 +205,10 @@ mh_status mh_score_seeds(mh_handle *h, const mh_index *idx, const uint8_t *d_que
 /* Tile choice for the length-bucketed persistent form. */
 typedef struct {
   uint32_t max_len;        /* bucket holds reads with length <= max_len */
-  uint32_t block;          /* threads per workgroup: 64, 256 or 1024 */
-  uint32_t entries_cap;    /* LDS capacity in bucket entries (power of two) */
+  uint32_t block;          /* threads per query: 16 or 32 (lane teams, 64 / block
+                              queries per one-wavefront workgroup) or 64..1024
+                              (a workgroup per query), a power of two */
+  uint32_t entries_cap;    /* LDS capacity in bucket entries (power of two, >= 32) */
   uint32_t grid;           /* persistent workgroups (0: derive from occupancy) */
 } mh_tile;
 

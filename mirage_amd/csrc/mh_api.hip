@@ -108,8 +108,9 @@ bool valid_tiles(const mh_tile *t, uint32_t n, uint32_t max_len) {
   if (n == 0 || n > 8) return false;
   for (uint32_t i = 0; i < n; i++) {
     const uint32_t b = t[i].block;
-    if (b != 64 && b != 128 && b != 256 && b != 512 && b != 1024) return false;
-    if (t[i].entries_cap < 64 || (t[i].entries_cap & (t[i].entries_cap - 1))) return false;
+    if (b != 16 && b != 32 && b != 64 && b != 128 && b != 256 && b != 512 && b != 1024)
+      return false;
+    if (t[i].entries_cap < 32 || (t[i].entries_cap & (t[i].entries_cap - 1))) return false;
     if (i && t[i].max_len <= t[i - 1].max_len) return false;
   }
   return t[n - 1].max_len >= max_len;
@@ -267,8 +268,8 @@ static mh_status classify_on(mh_handle *h, Exec &x, const mh_index *idx, const u
         hipGraphExecDestroy(x.graph);
         x.graph = nullptr;
       }
-      st = ensure_scratch(h, x, classify_scratch_bytes_f(n_reads, h->max_read_len,
-                                                         idx->threshold / 2, x.tie_factor));
+      st = ensure_scratch(h, x, classify_scratch_bytes(h, n_reads, idx->threshold / 2,
+                                                       x.tie_factor, a.flags));
       if (st) return st;
       hipGraph_t g = nullptr;
       MH_HIP(h, hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));

@@ -1,11 +1,12 @@
 // Tile autotuner for the length-bucketed persistent form.
 //
 // For every length bucket of the handle's tile table, each candidate
-// (threads per workgroup, LDS entry capacity) is timed on the caller's reads
-// (the whole call, so that work a small tile pushes to the overflow tiers is
-// counted); the fastest candidate is installed.  Results do not depend on the
-// tile: a query whose working set does not fit a tile's LDS goes to the
-// large-workspace kernel, so only speed changes.
+// (team size, LDS entry capacity) is timed on the caller's reads (the whole
+// call, so that work a small tile pushes to the overflow tiers is counted);
+// the fastest candidate is installed.  Team sizes 16 and 32 put several
+// queries in one wavefront and are tried for tiles of reads up to 1024
+// bases.  Results do not depend on the tile: a query whose working set does
+// not fit a tile's LDS goes to the overflow tiers, so only speed changes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,8 +35,8 @@ extern "C" mh_status mh_autotune(mh_handle *h, const mh_index *idx, const uint8_
     std::lock_guard<std::mutex> lk(h->mu);
     tiles = h->tiles.empty() ? default_tiles(h->max_read_len) : h->tiles;
   }
-  const uint32_t blocks[] = {64, 128, 256, 512};
-  const uint32_t ecaps[] = {64, 128, 256, 512, 1024, 2048, 4096};
+  const uint32_t blocks[] = {16, 32, 64, 128, 256, 512};
+  const uint32_t ecaps[] = {32, 64, 128, 256, 512, 1024, 2048, 4096};
   mh_status st = MH_OK;
   auto time_bucket = [&](const std::vector<mh_tile> &tt, uint32_t, float &ms) -> mh_status {
     std::vector<float> v;
@@ -71,6 +72,8 @@ extern "C" mh_status mh_autotune(mh_handle *h, const mh_index *idx, const uint8_
     for (uint32_t bl : blocks)
       for (uint32_t ec : ecaps) {
         if (st) break;
+        if (bl < 64 && (tiles[b].max_len > 1024 || ec > 128)) continue;
+        if (bl >= 64 && ec < 64) continue;
         mh_tile cand{tiles[b].max_len, bl, ec, 0};
         if (cand.block == best.block && cand.entries_cap == best.entries_cap) continue;
         std::vector<mh_tile> tt = tiles;

@@ -131,26 +131,99 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *red) {
   }
 }
 
+// A team processes one query.  BlockTeam<BS>: the whole workgroup, with its
+// wavefronts taking seeds in turn.  LaneTeam<L>: L lanes of a one-wavefront
+// workgroup, 64 / L queries side by side.  Its members exchange data through
+// cross-lane operations and LDS without workgroup barriers: the LDS
+// operations of a wavefront execute in order, and the fences keep the
+// compiler from moving memory accesses across a team synchronisation.
+template <int BS>
+struct BlockTeam {
+  static constexpr uint32_t kSize = BS;
+  static constexpr uint32_t kWorkers = BS / 64;  // wavefronts that take seeds
+  static constexpr uint32_t kWorkerLanes = 64;
+  static constexpr uint32_t kPerWave = 1;        // teams per workgroup
+  __device__ static uint32_t rank() { return threadIdx.x; }
+  __device__ static uint32_t index() { return 0; }
+  __device__ static uint32_t worker() { return threadIdx.x >> 6; }
+  __device__ static uint32_t wlane() { return threadIdx.x & 63; }
+  __device__ static uint64_t wballot(bool p) { return __ballot(p); }
+  __device__ static void sync() { __syncthreads(); }
+  __device__ static bool any(int x) { return __syncthreads_or(x) != 0; }
+  __device__ static uint32_t excl_scan(uint32_t v, uint32_t *red, uint32_t &tot) {
+    return block_excl_scan<BS>(v, red, tot);
+  }
+  __device__ static uint32_t max(uint32_t v, uint32_t *red) { return block_max<BS>(v, red); }
+  __device__ static uint32_t sum(uint32_t v, uint32_t *red) { return block_sum<BS>(v, red); }
+};
+
+template <int L>
+struct LaneTeam {
+  static_assert(L == 8 || L == 16 || L == 32, "lane team size");
+  static constexpr uint32_t kSize = L;
+  static constexpr uint32_t kWorkers = 1;
+  static constexpr uint32_t kWorkerLanes = L;
+  static constexpr uint32_t kPerWave = 64 / L;
+  static constexpr uint64_t kMask = (1ull << L) - 1;
+  __device__ static uint32_t rank() { return threadIdx.x & (L - 1); }
+  __device__ static uint32_t index() { return threadIdx.x / L; }
+  __device__ static uint32_t worker() { return 0; }
+  __device__ static uint32_t wlane() { return threadIdx.x & (L - 1); }
+  __device__ static uint64_t wballot(bool p) {
+    return (__ballot(p) >> (threadIdx.x & (64 - L))) & kMask;
+  }
+  __device__ static void sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  __device__ static bool any(int x) {
+    sync();
+    return wballot(x != 0) != 0;
+  }
+  __device__ static uint32_t excl_scan(uint32_t v, uint32_t *, uint32_t &tot) {
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < L; d <<= 1) {
+      const uint32_t t = __shfl_up(inc, d, L);
+      if (rank() >= (uint32_t)d) inc += t;
+    }
+    tot = __shfl(inc, L - 1, L);
+    return inc - v;
+  }
+  __device__ static uint32_t max(uint32_t v, uint32_t *) {
+#pragma unroll
+    for (int d = L / 2; d > 0; d >>= 1) v = ::max(v, (uint32_t)__shfl_xor(v, d, L));
+    return v;
+  }
+  __device__ static uint32_t sum(uint32_t v, uint32_t *) {
+#pragma unroll
+    for (int d = L / 2; d > 0; d >>= 1) v += (uint32_t)__shfl_xor(v, d, L);
+    return v;
+  }
+};
+
 __device__ __forceinline__ bool pair_less(uint64_t ka, uint32_t va, uint64_t kb, uint32_t vb) {
   return ka < kb || (ka == kb && va < vb);
 }
 
-// Ascending sort of (key, val) pairs over n (power of two) slots.  Up to 64
-// slots are ranked inside the first wavefront (each lane counts the smaller
-// elements through shuffles and writes its element to its rank); larger
-// arrays use a bitonic network in LDS.
-template <int BS>
+// Ascending sort of (key, val) pairs over n (power of two) slots.  Up to one
+// wavefront's worth of slots (64, or the team's lanes) are ranked through
+// shuffles: each lane counts the smaller elements and writes its element to
+// its rank.  Larger arrays use a bitonic network in LDS.
+template <class TM>
 __device__ void bitonic_sort(uint64_t *key, uint32_t *val, uint32_t n) {
-  if (n <= 64) {
-    __syncthreads();
-    if (threadIdx.x < 64) {
-      const uint32_t lane = threadIdx.x;
+  constexpr uint32_t W = TM::kWorkerLanes;
+  if (n <= W) {
+    TM::sync();
+    if (TM::worker() == 0) {
+      const uint32_t lane = TM::wlane();
       uint64_t k = lane < n ? key[lane] : ~0ull;
       uint32_t v = lane < n ? val[lane] : ~0u;
       uint32_t rank = 0;
       for (uint32_t u = 0; u < n; u++) {
-        const uint32_t klo = __shfl((uint32_t)k, u, 64), khi = __shfl((uint32_t)(k >> 32), u, 64);
-        const uint32_t vu = __shfl(v, u, 64);
+        const uint32_t klo = __shfl((uint32_t)k, u, W), khi = __shfl((uint32_t)(k >> 32), u, W);
+        const uint32_t vu = __shfl(v, u, W);
         const uint64_t ku = ((uint64_t)khi << 32) | klo;
         rank += (pair_less(ku, vu, k, v) || (ku == k && vu == v && u < lane)) ? 1u : 0u;
       }
@@ -160,14 +233,14 @@ __device__ void bitonic_sort(uint64_t *key, uint32_t *val, uint32_t n) {
         val[rank] = v;
       }
     }
-    __syncthreads();
+    TM::sync();
     return;
   }
   for (uint32_t size = 2; size <= n; size <<= 1) {
     for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-      __syncthreads();
+      TM::sync();
       const uint32_t lg = 31 - __clz(stride);
-      for (uint32_t t = threadIdx.x; t < (n >> 1); t += BS) {
+      for (uint32_t t = TM::rank(); t < (n >> 1); t += TM::kSize) {
         const uint32_t i = ((t >> lg) << (lg + 1)) + (t & (stride - 1));
         const uint32_t j = i + stride;
         const bool asc = (i & size) == 0;
@@ -183,7 +256,7 @@ __device__ void bitonic_sort(uint64_t *key, uint32_t *val, uint32_t n) {
       }
     }
   }
-  __syncthreads();
+  TM::sync();
 }
 
 __device__ __forceinline__ uint32_t next_pow2(uint32_t x) {
@@ -202,12 +275,17 @@ struct Ws {
   uint32_t *v2;        // ecap
   uint32_t *t1, *t2, *t3, *rref;  // ecap each, one contiguous region (4 * ecap)
   uint32_t *bitmap;    // qcap / 32 + 1
-  uint32_t *red;       // 64 words
+  uint32_t *red;       // kRedWords words: block-scan partials, then broadcasts
   unsigned long long *acc;  // 2
   unsigned long long *stats;  // kNumStats, per workgroup
 };
 
 __host__ __device__ inline size_t al8(size_t x) { return (x + 7) & ~size_t(7); }
+
+// red[0..15]: per-wavefront partials of block scans (up to 1024 threads);
+// red[kBcast..]: values one lane hands to the team.
+constexpr uint32_t kRedWords = 24;
+constexpr uint32_t kBcast = 16;
 
 __host__ __device__ inline size_t ws_bytes(const Caps &c) {
   size_t b = 0;
@@ -216,8 +294,13 @@ __host__ __device__ inline size_t ws_bytes(const Caps &c) {
   b += 2 * (al8(c.ecap * 8ull) + al8(c.ecap * 4ull));
   b += al8(c.ecap * 16ull);
   b += al8((c.qcap / 32 + 1) * 4ull);
-  b += 64 * 4 + 16 + 64;
+  b += kRedWords * 4 + 16 + 64;
   return b;
+}
+
+// Bytes between the workspaces of the teams of one workgroup.
+__host__ __device__ inline size_t ws_stride_lds(const Caps &c) {
+  return (ws_bytes(c) + 15) & ~size_t(15);
 }
 
 __device__ inline Ws ws_carve(unsigned char *base, const Caps &c) {
@@ -241,7 +324,7 @@ __device__ inline Ws ws_carve(unsigned char *base, const Caps &c) {
   w.t3 = w.t1 + 2ull * c.ecap;
   w.rref = w.t1 + 3ull * c.ecap;
   w.bitmap = (uint32_t *)take((c.qcap / 32 + 1) * 4ull);
-  w.red = (uint32_t *)take(64 * 4);
+  w.red = (uint32_t *)take(kRedWords * 4);
   w.acc = (unsigned long long *)take(16);
   w.stats = (unsigned long long *)take(64);
   return w;
@@ -253,8 +336,17 @@ struct Ctx {
   Reads rd;
   uint32_t flags;
   uint32_t max_len;  // longest read the handle accepts
+  // Probe records written by k_probe for this launch's strands (rec[q *
+  // rec_stride + p]), or null: the kernel hashes and looks up itself.
+  const uint64_t *rec;
+  uint32_t rec_stride;
   Outputs out;
 };
+
+// Probe record: lo << 24 | cnt of the probe's bucket; kRecAbsent when the
+// hash has no bucket; cnt kRecCntMax when the count does not fit 24 bits.
+constexpr uint64_t kRecAbsent = ~0ull;
+constexpr uint32_t kRecCntMax = 0xFFFFFFu;
 
 enum : uint32_t { kDone = 0, kOverflow = 1 };
 
@@ -283,11 +375,13 @@ __device__ __forceinline__ float matrix_score(uint64_t sum_r, uint64_t sum_c, ui
 
 // Head/tail extension of one seed by one wavefront (MatrixCPU.cpp:93-172).
 // Returns the matched length tot and the head length h; all lanes agree.
+template <class TM>
 __device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const uint8_t *rb,
                                             uint64_t ref_len, uint32_t sref, uint32_t rs,
                                             uint32_t lref, uint32_t re, int T, bool ultrafast,
                                             uint32_t &tot, uint32_t &head, uint32_t &compares) {
-  const uint32_t lane = threadIdx.x & 63;
+  constexpr uint32_t W = TM::kWorkerLanes;
+  const uint32_t lane = TM::wlane();
   const uint32_t mid = re - rs;
   uint32_t t = 0, h = 0;
   compares = 0;
@@ -298,10 +392,10 @@ __device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const 
     const uint64_t qroom = re < m ? (uint64_t)(m - re) : 0;
     tl = tl < qroom ? tl : qroom;
     t = (uint32_t)tl;
-    for (uint32_t b = 0; b < tl; b += 64) {
+    for (uint32_t b = 0; b < tl; b += W) {
       const uint32_t l = b + lane;
       const bool mm = l < tl && q[re + l] != rb[(uint64_t)lref + l];
-      const unsigned long long bal = __ballot(mm);
+      const unsigned long long bal = TM::wballot(mm);
       if (bal) {
         t = b + (uint32_t)__ffsll((long long)bal) - 1;
         break;
@@ -316,10 +410,10 @@ __device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const 
       if (hs > 0) {
         const uint32_t lb = (rs - 1) < (hs - 1) ? (rs - 1) : (hs - 1);
         h = lb;
-        for (uint32_t b = 0; b <= lb; b += 64) {
+        for (uint32_t b = 0; b <= lb; b += W) {
           const uint32_t l = b + lane;
           const bool mm = l <= lb && q[rs - 1 - l] != rb[sref - l];
-          const unsigned long long bal = __ballot(mm);
+          const unsigned long long bal = TM::wballot(mm);
           if (bal) {
             h = b + (uint32_t)__ffsll((long long)bal) - 1;
             break;
@@ -333,11 +427,12 @@ __device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const 
   head = h;
 }
 
+template <class TM>
 __device__ __forceinline__ void set_bits(uint32_t *bm, uint32_t a, uint32_t len) {
-  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t lane = TM::wlane();
   const uint32_t b = a + len;  // [a, b)
   const uint32_t w0 = a >> 5, w1 = (b - 1) >> 5;
-  for (uint32_t w = w0 + lane; w <= w1; w += 64) {
+  for (uint32_t w = w0 + lane; w <= w1; w += TM::kWorkerLanes) {
     uint32_t mask = 0xFFFFFFFFu;
     if (w == w0) mask &= 0xFFFFFFFFu << (a & 31);
     if (w == w1) mask &= 0xFFFFFFFFu >> (31 - ((b - 1) & 31));
@@ -347,20 +442,20 @@ __device__ __forceinline__ void set_bits(uint32_t *bm, uint32_t a, uint32_t len)
 
 // Scores one reference from its seeds: every wave takes seeds in turn.
 // get_seed(i, sref, sq, lref, lq) supplies seed i.  Returns on all threads.
-template <int BS, class GetSeed>
+template <class TM, class GetSeed>
 __device__ void score_ref(const uint8_t *q, uint32_t m, const uint8_t *rb, uint64_t ref_len,
                           uint32_t n_seeds, int T, bool ultrafast, const Ws &ws,
                           GetSeed get_seed, uint64_t &sum_r, uint64_t &sum_c, uint32_t &hits) {
-  constexpr uint32_t NW = BS / 64;
-  const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr uint32_t NW = TM::kWorkers, BS = TM::kSize;
+  const uint32_t wid = TM::worker(), lane = TM::wlane();
   const uint32_t nw = (m + 31) >> 5;
-  __syncthreads();
-  for (uint32_t w = threadIdx.x; w < nw; w += BS) ws.bitmap[w] = 0;
-  if (threadIdx.x == 0) {
+  TM::sync();
+  for (uint32_t w = TM::rank(); w < nw; w += BS) ws.bitmap[w] = 0;
+  if (TM::rank() == 0) {
     ws.acc[0] = 0;
     ws.acc[1] = 0;
   }
-  __syncthreads();
+  TM::sync();
   unsigned long long sr = 0, sc = 0, ncmp = 0;
   for (uint32_t i = wid; i < n_seeds; i += NW) {
     uint32_t sref, sq, lref, lq;
@@ -368,12 +463,12 @@ __device__ void score_ref(const uint8_t *q, uint32_t m, const uint8_t *rb, uint6
     const uint32_t rs = sq, re = lq;
     if (rs > re) continue;  // MatrixCPU.cpp:57-58
     uint32_t tot, h, cmp;
-    extend_seed(q, m, rb, ref_len, sref, rs, lref, re, T, ultrafast, tot, h, cmp);
+    extend_seed<TM>(q, m, rb, ref_len, sref, rs, lref, re, T, ultrafast, tot, h, cmp);
     ncmp += cmp;
     if (tot >= (uint32_t)T) {  // MatrixCPU.cpp:178-184
       sc += tot;
       sr += (unsigned long long)tot * (tot + 1) / 2;
-      set_bits(ws.bitmap, rs - h, tot);
+      set_bits<TM>(ws.bitmap, rs - h, tot);
     }
   }
   if (lane == 0 && sc) {
@@ -381,17 +476,18 @@ __device__ void score_ref(const uint8_t *q, uint32_t m, const uint8_t *rb, uint6
     atomicAdd(&ws.acc[1], sc);
   }
   if (lane == 0 && ncmp && ws.stats) atomicAdd(&ws.stats[kExt], ncmp);
-  __syncthreads();
+  TM::sync();
   uint32_t pc = 0;
-  for (uint32_t w = threadIdx.x; w < nw; w += BS) pc += __popc(ws.bitmap[w]);
-  hits = block_sum<BS>(pc, ws.red);
-  __syncthreads();
+  for (uint32_t w = TM::rank(); w < nw; w += BS) pc += __popc(ws.bitmap[w]);
+  hits = TM::sum(pc, ws.red);
+  TM::sync();
   sum_r = ws.acc[0];
   sum_c = ws.acc[1];
 }
 
+template <class TM>
 __device__ __forceinline__ void write_query(const Ctx &c, uint32_t q, const QueryOut &r) {
-  if (threadIdx.x == 0) c.out.q[q] = r;
+  if (TM::rank() == 0) c.out.q[q] = r;
 }
 
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
@@ -422,7 +518,7 @@ __device__ __forceinline__ uint32_t map_slot(uint32_t *keys, uint32_t mask, uint
 // Returns, per element, the length of its run in `ids` (distinct hits; a
 // repeat of one hit in a bucket does not lengthen a run, Analyser.cpp:2057-
 // 2058), using `len` as scratch.  All threads get the maximum.
-template <int BS, int KS, int RS>
+template <int KS, int RS>
 __device__ __forceinline__ uint32_t run_flags(const uint64_t *key, uint32_t t) {
   if (t == 0) return 3u;
   const uint64_t p = key[t] >> KS, pp = key[t - 1] >> KS;
@@ -430,33 +526,34 @@ __device__ __forceinline__ uint32_t run_flags(const uint64_t *key, uint32_t t) {
   return ((!same || (p != pp && p != pp + 1)) ? 1u : 0u) | ((!same || p != pp) ? 2u : 0u);
 }
 
-template <int BS, int KS, int RS>
+template <class TM, int KS, int RS>
 __device__ uint32_t run_lengths(const Ws &ws, const uint64_t *key, uint32_t n, uint32_t *ids,
                                 uint32_t *len) {
-  const uint32_t tid = threadIdx.x;
+  constexpr uint32_t BS = TM::kSize;
+  const uint32_t tid = TM::rank();
   uint32_t nruns = 0;
   for (uint32_t base = 0; base < n; base += BS) {
     const uint32_t t = base + tid;
-    const uint32_t st = t < n ? (run_flags<BS, KS, RS>(key, t) & 1u) : 0u;
+    const uint32_t st = t < n ? (run_flags<KS, RS>(key, t) & 1u) : 0u;
     uint32_t tot;
-    const uint32_t ex = block_excl_scan<BS>(st, ws.red, tot);
+    const uint32_t ex = TM::excl_scan(st, ws.red, tot);
     if (t < n) ids[t] = nruns + ex + st - 1;
     nruns += tot;
   }
-  __syncthreads();
+  TM::sync();
   for (uint32_t r = tid; r < nruns; r += BS) len[r] = 0;
-  __syncthreads();
+  TM::sync();
   for (uint32_t t = tid; t < n; t += BS)
-    if (run_flags<BS, KS, RS>(key, t) & 2u) atomicAdd(&len[ids[t]], 1u);
-  __syncthreads();
+    if (run_flags<KS, RS>(key, t) & 2u) atomicAdd(&len[ids[t]], 1u);
+  TM::sync();
   uint32_t mx = 0;
   for (uint32_t t = tid; t < n; t += BS) {
     const uint32_t L = len[ids[t]];
     mx = max(mx, L);
     ids[t] = L;  // per element from here on
   }
-  const uint32_t m = block_max<BS>(mx, ws.red);
-  __syncthreads();
+  const uint32_t m = TM::max(mx, ws.red);
+  TM::sync();
   return m;
 }
 
@@ -466,23 +563,24 @@ __device__ uint32_t run_lengths(const Ws &ws, const uint64_t *key, uint32_t n, u
 // measure their runs.  The reference map has its values in t1..t2 and its
 // keys in t3..rref (2E slots each); the keys are free again after the
 // inserts and hold the run scratch.
-template <int BS>
+template <class TM>
 __device__ void filter_hashed(const Ws &ws, uint32_t E, uint32_t &nP) {
-  const uint32_t tid = threadIdx.x;
+  constexpr uint32_t BS = TM::kSize;
+  const uint32_t tid = TM::rank();
   const uint32_t cap = next_pow2(2 * E), mask = cap - 1;
   uint32_t *rval = ws.t1, *rkeys = ws.t3;
   for (uint32_t i = tid; i < cap; i += BS) {
     rkeys[i] = kEmpty;
     rval[i] = 0;
   }
-  __syncthreads();
+  TM::sync();
   for (uint32_t e = tid; e < E; e += BS) {
     const uint32_t r = (uint32_t)(ws.k1[e] >> 32);
     const uint32_t slot = map_slot(rkeys, mask, r);
     atomicAdd(&rval[slot], 1u);
     ws.v2[e] = slot;  // map slot of each pair
   }
-  __syncthreads();
+  TM::sync();
   // Pairs of references with two or more entries -> k2 as
   // ((ref << 12 | hit) << 32 | slot).
   uint32_t nm = 0;
@@ -496,24 +594,24 @@ __device__ void filter_hashed(const Ws &ws, uint32_t E, uint32_t &nP) {
       multi = rval[slot] >= 2 ? 1u : 0u;
     }
     uint32_t tot;
-    const uint32_t ex = block_excl_scan<BS>(multi, ws.red, tot);
+    const uint32_t ex = TM::excl_scan(multi, ws.red, tot);
     if (multi)
       ws.k2[nm + ex] =
           ((uint64_t)(((uint32_t)(key >> 32) << 12) | (uint32_t)key) << 32) | slot;
     nm += tot;
   }
-  __syncthreads();
+  TM::sync();
   // Every reference starts with a run of 1; the sorted pass raises it.
   for (uint32_t i = tid; i < cap; i += BS) rval[i] = 1;
   const uint32_t n2 = next_pow2(nm);
   for (uint32_t e = nm + tid; e < n2; e += BS) ws.k2[e] = ~0ull;
-  __syncthreads();
+  TM::sync();
   uint32_t maxrun = 0;
   if (nm) {
-    bitonic_sort<BS>(ws.k2, ws.t3, n2);  // t3: value scratch, unused
-    maxrun = run_lengths<BS, 32, 12>(ws, ws.k2, nm, ws.t3, ws.rref);
+    bitonic_sort<TM>(ws.k2, ws.t3, n2);  // t3: value scratch, unused
+    maxrun = run_lengths<TM, 32, 12>(ws, ws.k2, nm, ws.t3, ws.rref);
     for (uint32_t t = tid; t < nm; t += BS) atomicMax(&rval[(uint32_t)ws.k2[t]], ws.t3[t]);
-    __syncthreads();
+    TM::sync();
   }
   const uint32_t cut = maxrun >= 2 ? maxrun - 1 : 1;
   // Kept pairs, compacted in place into k2/v2 (position <= source index;
@@ -530,8 +628,8 @@ __device__ void filter_hashed(const Ws &ws, uint32_t E, uint32_t &nP) {
       keep = rval[ws.v2[e]] >= cut ? 1u : 0u;
     }
     uint32_t tot;
-    const uint32_t ex = block_excl_scan<BS>(keep, ws.red, tot);
-    __syncthreads();
+    const uint32_t ex = TM::excl_scan(keep, ws.red, tot);
+    TM::sync();
     if (keep) {
       ws.k2[nkeep + ex] = (key & 0xFFFFFFFF00000000ull) | pos;
       ws.v2[nkeep + ex] = j;
@@ -539,46 +637,47 @@ __device__ void filter_hashed(const Ws &ws, uint32_t E, uint32_t &nP) {
     nkeep += tot;
   }
   nP = nkeep;
-  __syncthreads();
+  TM::sync();
 }
 
 // Successive-occurrence filter by sorting every pair by (ref, hit); used
 // when (ref, hit) does not pack into 32 bits.
-template <int BS>
+template <class TM>
 __device__ void filter_sorted(const Ws &ws, uint32_t E, uint32_t &nP) {
-  const uint32_t tid = threadIdx.x;
+  constexpr uint32_t BS = TM::kSize;
+  const uint32_t tid = TM::rank();
   const uint32_t n2 = next_pow2(E);
   for (uint32_t e = E + tid; e < n2; e += BS) {
     ws.k1[e] = ~0ull;
     ws.v1[e] = ~0u;
   }
-  bitonic_sort<BS>(ws.k1, ws.v1, n2);
-  const uint32_t maxrun = run_lengths<BS, 0, 32>(ws, ws.k1, E, ws.t2, ws.t3);
+  bitonic_sort<TM>(ws.k1, ws.v1, n2);
+  const uint32_t maxrun = run_lengths<TM, 0, 32>(ws, ws.k1, E, ws.t2, ws.t3);
   const uint32_t cut = maxrun >= 2 ? maxrun - 1 : 1;
   // longest run per reference segment
   for (uint32_t t = tid; t < E; t += BS)
     ws.t1[t] = (t == 0 || (ws.k1[t - 1] >> 32) != (ws.k1[t] >> 32)) ? 1u : 0u;
-  __syncthreads();
+  TM::sync();
   uint32_t nseg = 0;
   for (uint32_t base = 0; base < E; base += BS) {
     const uint32_t t = base + tid;
     const uint32_t st = t < E ? ws.t1[t] : 0u;
     uint32_t tot;
-    const uint32_t ex = block_excl_scan<BS>(st, ws.red, tot);
+    const uint32_t ex = TM::excl_scan(st, ws.red, tot);
     if (t < E) ws.t3[t] = nseg + ex + st - 1;
     nseg += tot;
   }
-  __syncthreads();
+  TM::sync();
   for (uint32_t r = tid; r < nseg; r += BS) ws.rref[r] = 0;
-  __syncthreads();
+  TM::sync();
   for (uint32_t t = tid; t < E; t += BS) atomicMax(&ws.rref[ws.t3[t]], ws.t2[t]);
-  __syncthreads();
+  TM::sync();
   uint32_t nkeep = 0;
   for (uint32_t base = 0; base < E; base += BS) {
     const uint32_t t = base + tid;
     const uint32_t keep = (t < E && ws.rref[ws.t3[t]] >= cut) ? 1u : 0u;
     uint32_t tot;
-    const uint32_t ex = block_excl_scan<BS>(keep, ws.red, tot);
+    const uint32_t ex = TM::excl_scan(keep, ws.red, tot);
     if (keep) {
       const uint64_t key = ws.k1[t];
       ws.k2[nkeep + ex] = (key & 0xFFFFFFFF00000000ull) | ws.v1[t];
@@ -587,17 +686,16 @@ __device__ void filter_sorted(const Ws &ws, uint32_t E, uint32_t &nP) {
     nkeep += tot;
   }
   nP = nkeep;
-  __syncthreads();
+  TM::sync();
 }
 
 // Processes query q.  Returns kOverflow (uniformly) when the working set
 // exceeds the workspace capacities; nothing is written in that case.
-template <int BS>
+template <class TM>
 __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, uint32_t q,
                                   bool last_resort) {
-  constexpr uint32_t NW = BS / 64;
-  (void)NW;
-  const uint32_t tid = threadIdx.x;
+  constexpr uint32_t BS = TM::kSize;
+  const uint32_t tid = TM::rank();
   const uint32_t read = q >> 1, strand = q & 1;
   const IndexView &ix = c.ix;
   const int T = ix.T, s = ix.s, k = ix.k;
@@ -613,13 +711,13 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
   res.pad = 0;
   if (m64 > c.max_len) {
     res.status = MH_READ_TOO_LONG;
-    write_query(c, q, res);
+    write_query<TM>(c, q, res);
     return kDone;
   }
   if (m64 > cap.qcap) {
     if (!last_resort) return kOverflow;
     res.status = MH_READ_TOO_LONG;
-    write_query(c, q, res);
+    write_query<TM>(c, q, res);
     return kDone;
   }
   const uint32_t m = (uint32_t)m64;
@@ -633,43 +731,55 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
     bad |= (cc == 0);
     ws.q[x] = strand ? cc : ch;
   }
-  bad = __syncthreads_or(bad);
+  bad = TM::any(bad);
   if (bad) {
     res.status = MH_READ_BADBASE;
-    write_query(c, q, res);
+    write_query<TM>(c, q, res);
     return kDone;
   }
   if (m <= (uint32_t)T) {  // no well-defined probe (the reference reads past the end)
-    write_query(c, q, res);
+    write_query<TM>(c, q, res);
     return kDone;
   }
   const uint32_t nprobe = (m - T + s - 1) / s;
   if (nprobe > cap.pcap) {
     if (!last_resort) return kOverflow;
     res.status = MH_READ_CAPACITY;
-    write_query(c, q, res);
+    write_query<TM>(c, q, res);
     return kDone;
   }
 
   // Probe every s-th window; compact present hashes into the hit list.
-  uint32_t nhits = 0;
+  // With probe records, the lookups were made beforehand by k_probe.
+  const bool use_rec = c.rec && nprobe <= c.rec_stride;
+  uint32_t nhits = 0, big = 0;
   for (uint32_t base = 0; base < nprobe; base += BS) {
     const uint32_t p = base + tid;
     uint32_t present = 0, cnt = 0;
     uint64_t lo = 0;
     if (p < nprobe) {
-      const uint32_t h = hash28(ws.q + p * s, k);
-      if (h == ix.empty_hash) {
-        present = 1;
+      if (use_rec) {
+        const uint64_t r = c.rec[(uint64_t)q * c.rec_stride + p];
+        present = r != kRecAbsent;
+        if (present) {
+          lo = r >> 24;
+          cnt = (uint32_t)(r & kRecCntMax);
+          big |= cnt == kRecCntMax ? 1u : 0u;
+        }
       } else {
-        lo = ix.dir[h];
-        const uint64_t hi = ix.dir[h + 1];
-        present = hi > lo;
-        cnt = (uint32_t)(hi - lo);
+        const uint32_t h = hash28(ws.q + p * s, k);
+        if (h == ix.empty_hash) {
+          present = 1;
+        } else {
+          lo = ix.dir[h];
+          const uint64_t hi = ix.dir[h + 1];
+          present = hi > lo;
+          cnt = (uint32_t)(hi - lo);
+        }
       }
     }
     uint32_t tot;
-    const uint32_t idx = block_excl_scan<BS>(present, ws.red, tot);
+    const uint32_t idx = TM::excl_scan(present, ws.red, tot);
     if (present) {
       ws.hit_j[nhits + idx] = p * s;
       ws.hit_lo[nhits + idx] = lo;
@@ -677,22 +787,23 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
     }
     nhits += tot;
   }
-  __syncthreads();
+  TM::sync();
   uint32_t E = 0;
   for (uint32_t base = 0; base < nhits; base += BS) {
     const uint32_t i = base + tid;
     const uint32_t cnt = i < nhits ? ws.hit_base[i] : 0;
     uint32_t tot;
-    const uint32_t ex = block_excl_scan<BS>(cnt, ws.red, tot);
+    const uint32_t ex = TM::excl_scan(cnt, ws.red, tot);
     if (i < nhits) ws.hit_base[i] = E + ex;
     E += tot;
   }
   if (tid == 0) ws.hit_base[nhits] = E;
-  __syncthreads();
+  if (use_rec && TM::any(big)) E = 0xFFFFFFFFu;  // a saturated count: overflow
+  TM::sync();
   if (E > cap.ecap) {
     if (!last_resort) return kOverflow;
     res.status = MH_READ_CAPACITY;
-    write_query(c, q, res);
+    write_query<TM>(c, q, res);
     return kDone;
   }
   if (tid == 0) {
@@ -701,7 +812,7 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
     ws.stats[kEntries] += E;
   }
   if (nhits == 0 || E == 0) {  // Analyser.cpp:1989-1990, :2012-2013
-    write_query(c, q, res);
+    write_query<TM>(c, q, res);
     return kDone;
   }
 
@@ -717,7 +828,7 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
     ws.k1[e] = ((uint64_t)en.x << 32) | lo;
     ws.v1[e] = en.y;
   }
-  __syncthreads();
+  TM::sync();
 
   uint32_t nP = 0, nranks = 0, nchains = 0, G = 0;
   bool direct = false;
@@ -750,9 +861,9 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
         rstar = r;
       }
     }
-    const uint32_t nfound = block_sum<BS>(found, ws.red);
+    const uint32_t nfound = TM::sum(found, ws.red);
     if (nfound == 1) {
-      const uint32_t rs_ = block_max<BS>(found ? rstar : 0u, ws.red);
+      const uint32_t rs_ = TM::max(found ? rstar : 0u, ws.red);
       // Any other reference with a run of N - 1 occurs in buckets 1..N-2.
       int other = 0;
       for (uint32_t e = ws.hit_base[1] + tid; e < ws.hit_base[2]; e += BS) {
@@ -767,7 +878,7 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
         }
         other |= all ? 1 : 0;
       }
-      direct = __syncthreads_or(other) == 0;
+      direct = TM::any(other) == 0;
       if (direct) {
         // P = r*'s entries in hit order (already sorted by refPos, qPos).
         if (tid == 0) {
@@ -791,7 +902,7 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
         nchains = 1;
         G = N;
       }
-      __syncthreads();
+      TM::sync();
     }
   }
 
@@ -810,8 +921,8 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
           link |= ((uint32_t)(ws.k1[f] >> 32) == r && ws.v1[f] == p) ? 1 : 0;
       }
     }
-    if (!__syncthreads_or(link)) {
-      write_query(c, q, res);
+    if (!TM::any(link)) {
+      write_query<TM>(c, q, res);
       return kDone;
     }
   }
@@ -823,28 +934,28 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
     // pair of a reference whose run reaches the cut is kept.  Result: kept
     // pairs in k2/v2 as (ref << 32 | refPos, qPos), unsorted.
     if (nhits < (1u << 12) && ix.n_refs < (1u << 20))
-      filter_hashed<BS>(ws, E, nP);
+      filter_hashed<TM>(ws, E, nP);
     else
-      filter_sorted<BS>(ws, E, nP);
+      filter_sorted<TM>(ws, E, nP);
     // Sort by (reference, refPos, qPos) (Analyser.cpp:1024-1059).
     const uint32_t p2 = next_pow2(nP);
-    __syncthreads();
+    TM::sync();
     for (uint32_t e = nP + tid; e < p2; e += BS) {
       ws.k2[e] = ~0ull;
       ws.v2[e] = ~0u;
     }
-    bitonic_sort<BS>(ws.k2, ws.v2, p2);
+    bitonic_sort<TM>(ws.k2, ws.v2, p2);
     // Candidate references in ascending order: rank = index among the
     // distinct references; k2 becomes rank << 32 | refPos.
     for (uint32_t t = tid; t < nP; t += BS)
       ws.t1[t] = (t == 0 || (ws.k2[t - 1] >> 32) != (ws.k2[t] >> 32)) ? 1u : 0u;
-    __syncthreads();
+    TM::sync();
     uint32_t nrank = 0;
     for (uint32_t base = 0; base < nP; base += BS) {
       const uint32_t t = base + tid;
       const uint32_t st = t < nP ? ws.t1[t] : 0u;
       uint32_t tot;
-      const uint32_t ex = block_excl_scan<BS>(st, ws.red, tot);
+      const uint32_t ex = TM::excl_scan(st, ws.red, tot);
       if (t < nP) {
         const uint64_t key = ws.k2[t];
         const uint32_t rank = nrank + ex + st - 1;
@@ -854,7 +965,7 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
       nrank += tot;
     }
     nranks = nrank;
-    __syncthreads();
+    TM::sync();
     // Chains: neighbours link when both coordinates step by s
     // (Analyser.cpp:2351-2353, u32 arithmetic).
     for (uint32_t t = tid; t < nP; t += BS) {
@@ -868,18 +979,18 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
       }
       ws.t1[t] = st;
     }
-    __syncthreads();
+    TM::sync();
     uint32_t nch = 0;
     for (uint32_t base = 0; base < nP; base += BS) {
       const uint32_t t = base + tid;
       const uint32_t st = t < nP ? ws.t1[t] : 0u;
       uint32_t tot;
-      const uint32_t ex = block_excl_scan<BS>(st, ws.red, tot);
+      const uint32_t ex = TM::excl_scan(st, ws.red, tot);
       if (t < nP) ws.t2[t] = nch + ex + st - 1;  // chain id
       nch += tot;
     }
     nchains = nch;
-    __syncthreads();
+    TM::sync();
     uint32_t *cfl = (uint32_t *)ws.k1;  // chain first/last element
     for (uint32_t t = tid; t < nP; t += BS) {
       const uint32_t cid = ws.t2[t];
@@ -889,20 +1000,20 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
       if (t == 0 || (uint32_t)(ws.k2[t - 1] >> 32) != rank) ws.v1[rank] = cid;
     }
     for (uint32_t r = tid; r < nranks; r += BS) ws.t3[r] = 0;
-    __syncthreads();
+    TM::sync();
     for (uint32_t cid = tid; cid < nchains; cid += BS) {
       const uint32_t f = cfl[2 * cid], l = cfl[2 * cid + 1];
       atomicMax(&ws.t3[(uint32_t)(ws.k2[f] >> 32)], l - f + 1);
     }
-    __syncthreads();
+    TM::sync();
     uint32_t gm = 0;
     for (uint32_t r = tid; r < nranks; r += BS) gm = max(gm, ws.t3[r]);
-    G = block_max<BS>(gm, ws.red);
-    __syncthreads();
+    G = TM::max(gm, ws.red);
+    TM::sync();
   }
 
   if (G < 2) {  // Analyser.cpp:2398-2403: no seeds, every candidate scores -2
-    write_query(c, q, res);
+    write_query<TM>(c, q, res);
     return kDone;
   }
 
@@ -933,7 +1044,7 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
     };
     uint64_t sum_r, sum_c;
     uint32_t hits;
-    score_ref<BS>(ws.q, m, rb, ref_len, c1 - c0, T, ultrafast, ws, seed, sum_r, sum_c, hits);
+    score_ref<TM>(ws.q, m, rb, ref_len, c1 - c0, T, ultrafast, ws, seed, sum_r, sum_c, hits);
     const float score = matrix_score(sum_r, sum_c, hits, m);
     float rel = -1.0f;
     uint32_t valid = 0, lut_bad = 0;
@@ -966,7 +1077,7 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
       }
     }
   }
-  __syncthreads();
+  TM::sync();
   if (tid == 0) {
     ws.stats[kSeeds] += seeds_total;
     ws.stats[kCands] += seeded;
@@ -980,8 +1091,8 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
       nv++;
       hb = max(hb, __float_as_uint(rel_of[r]));
     }
-  const uint32_t n_cand = block_sum<BS>(nv, ws.red);
-  const uint32_t best = block_max<BS>(hb, ws.red);
+  const uint32_t n_cand = TM::sum(nv, ws.red);
+  const uint32_t best = TM::max(hb, ws.red);
   res.n_cand = n_cand;
   if (n_cand > 0) {
     res.highest = __uint_as_float(best);
@@ -990,11 +1101,11 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
       const uint32_t r = base + tid;
       const uint32_t tie = (r < nranks && valid_of[r] && __float_as_uint(rel_of[r]) == best);
       uint32_t tot;
-      const uint32_t ex = block_excl_scan<BS>(tie, ws.red, tot);
+      const uint32_t ex = TM::excl_scan(tie, ws.red, tot);
       if (tie) ws.t3[ntop + ex] = ws.rref[r];  // t3 is free once seeding is done
       ntop += tot;
     }
-    __syncthreads();
+    TM::sync();
     res.n_top = ntop;
     if (ntop <= (uint32_t)kTopInline) {
       // constant indices keep res in registers
@@ -1003,90 +1114,107 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
     } else {
       if (tid == 0) {
         const unsigned long long off = atomicAdd(&c.out.ctr->tie_used, (unsigned long long)ntop);
-        ws.red[40] = (uint32_t)off;
-        ws.red[41] = (off + ntop <= c.out.tie_cap) ? 1u : 0u;
+        ws.red[kBcast] = (uint32_t)off;
+        ws.red[kBcast + 1] = (off + ntop <= c.out.tie_cap) ? 1u : 0u;
       }
-      __syncthreads();
-      res.tie_off = ws.red[40];
-      if (ws.red[41])
+      TM::sync();
+      res.tie_off = ws.red[kBcast];
+      if (ws.red[kBcast + 1])
         for (uint32_t i = tid; i < ntop; i += BS) c.out.ties[res.tie_off + i] = ws.t3[i];
     }
   }
-  write_query(c, q, res);
-  __syncthreads();
+  write_query<TM>(c, q, res);
+  TM::sync();
   return kDone;
 }
 
+template <class TM>
 __device__ __forceinline__ void stats_begin(const Ws &ws) {
-  if (threadIdx.x < kNumStats) ws.stats[threadIdx.x] = 0;
-  __syncthreads();
+  if (TM::rank() < kNumStats) ws.stats[TM::rank()] = 0;
+  TM::sync();
 }
 
-__device__ __forceinline__ void stats_flush(const Ws &ws, DevCounters *ctr) {
-  __syncthreads();
-  if (threadIdx.x < kNumStats && ws.stats[threadIdx.x])
-    atomicAdd(&ctr->stats[blockIdx.x & (kStatSlots - 1)][threadIdx.x], ws.stats[threadIdx.x]);
+// Adds the team's statistics to slot `slot` of the device counters.
+template <class TM>
+__device__ __forceinline__ void stats_flush(const Ws &ws, DevCounters *ctr, uint32_t slot) {
+  TM::sync();
+  if (TM::rank() < kNumStats && ws.stats[TM::rank()])
+    atomicAdd(&ctr->stats[slot & (kStatSlots - 1)][TM::rank()], ws.stats[TM::rank()]);
 }
 
-constexpr uint32_t kQueueChunk = 8;
+template <class TM>
+__device__ __forceinline__ void pass_on(const Ctx &c, uint32_t q) {
+  if (TM::rank() == 0) {
+    const unsigned long long slot = atomicAdd(c.out.overflow_cnt, 1ull);
+    if (slot < c.out.overflow_cap) c.out.overflow[slot] = q;
+  }
+}
 
-// One query per workgroup (plain form) or a persistent loop that takes
-// queries from a queue kQueueChunk at a time.
-template <int BS>
-__global__ __launch_bounds__(BS) void k_classify(Ctx c, Caps cap, const uint32_t *qlist,
-                                                 const unsigned long long *qstart, uint32_t nq,
-                                                 const unsigned long long *nq_dev,
-                                                 unsigned long long *head) {
+// Most queries a team takes from its queue at a time.
+constexpr uint32_t kQueueChunk = 16;
+
+// Length-tile kernel.  Each team has its own workspace in LDS.  Plain form:
+// one query per team; batched form: a persistent loop over a queue of
+// queries, taken a chunk per workgroup at a time: up to kQueueChunk per team
+// while the queue holds at least four chunks per team, one per team for
+// short queues (the overflow tiers).  Queries that do not fit are passed on
+// to the next tier.
+template <class TM>
+__global__ __launch_bounds__(TM::kSize * TM::kPerWave) void k_classify(
+    Ctx c, Caps cap, const uint32_t *qlist, const unsigned long long *qstart, uint32_t nq,
+    const unsigned long long *nq_dev, unsigned long long *head) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const Ws ws = ws_carve(smem, cap);
+  constexpr uint32_t G = TM::kPerWave;
+  const uint32_t team = TM::index();
+  const Ws ws = ws_carve(smem + team * ws_stride_lds(cap), cap);
+  __shared__ uint32_t s_base;
   if (qlist && qstart) qlist += *qstart;
-  stats_begin(ws);
+  stats_begin<TM>(ws);
+  const uint32_t slot = blockIdx.x * G + team;
   if (head == nullptr) {
-    const uint32_t i = blockIdx.x;
-    if (i < nq) {
-      const uint32_t q = qlist ? qlist[i] : i;
-      if (process_query<BS>(c, ws, cap, q, false) == kOverflow && threadIdx.x == 0) {
-        const unsigned long long slot = atomicAdd(c.out.overflow_cnt, 1ull);
-        if (slot < c.out.overflow_cap) c.out.overflow[slot] = q;
-      }
+    if (slot < nq) {
+      const uint32_t q = qlist ? qlist[slot] : slot;
+      if (process_query<TM>(c, ws, cap, q, false) == kOverflow) pass_on<TM>(c, q);
     }
-    stats_flush(ws, c.out.ctr);
+    stats_flush<TM>(ws, c.out.ctr, slot);
     return;
   }
   const unsigned long long n = nq_dev ? *nq_dev : nq;
+  const unsigned long long per_team =
+      n / ((unsigned long long)gridDim.x * G * 4ull * kQueueChunk) ? kQueueChunk
+      : max(1ull, n / ((unsigned long long)gridDim.x * G * 4ull));
+  const uint32_t chunk = (uint32_t)per_team * G;
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) {
-      const unsigned long long i = atomicAdd(head, (unsigned long long)kQueueChunk);
-      ws.red[48] = i < n ? (uint32_t)i : 0xFFFFFFFFu;
+      const unsigned long long i = atomicAdd(head, (unsigned long long)chunk);
+      s_base = i < n ? (uint32_t)i : 0xFFFFFFFFu;
     }
     __syncthreads();
-    const uint32_t base = ws.red[48];
+    const uint32_t base = s_base;
     if (base == 0xFFFFFFFFu) break;
-    const uint32_t end = (uint32_t)min((unsigned long long)base + kQueueChunk, n);
-    for (uint32_t i = base; i < end; i++) {
+    const uint32_t end = (uint32_t)min((unsigned long long)base + chunk, n);
+    for (uint32_t i = base + team; i < end; i += G) {
       const uint32_t q = qlist ? qlist[i] : i;
-      if (process_query<BS>(c, ws, cap, q, false) == kOverflow && threadIdx.x == 0) {
-        const unsigned long long slot = atomicAdd(c.out.overflow_cnt, 1ull);
-        if (slot < c.out.overflow_cap) c.out.overflow[slot] = q;
-      }
+      if (process_query<TM>(c, ws, cap, q, false) == kOverflow) pass_on<TM>(c, q);
     }
   }
-  stats_flush(ws, c.out.ctr);
+  stats_flush<TM>(ws, c.out.ctr, slot);
 }
 
-// Queries that overflowed both LDS tiers, with the workspace in HBM: one
+// Queries that overflowed the LDS tiers, with the workspace in HBM: one
 // workspace per workgroup, queries taken one at a time from `list`.  The
 // last resort flags what it cannot hold; the tier before it passes such
-// queries on through c.out.overflow.
+// queries on.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_classify_hbm(Ctx c, Caps cap, unsigned char *scratch,
                                                      size_t ws_stride, const uint32_t *list,
                                                      const unsigned long long *count,
                                                      unsigned long long *head, int last_resort) {
+  using TM = BlockTeam<BS>;
   const Ws ws = ws_carve(scratch + blockIdx.x * ws_stride, cap);
   __shared__ uint32_t slot;
-  stats_begin(ws);
+  stats_begin<TM>(ws);
   for (;;) {
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1097,12 +1225,51 @@ __global__ __launch_bounds__(BS) void k_classify_hbm(Ctx c, Caps cap, unsigned c
     __syncthreads();
     if (slot == 0xFFFFFFFFu) break;
     const uint32_t q = list[slot];
-    if (process_query<BS>(c, ws, cap, q, last_resort != 0) == kOverflow && threadIdx.x == 0) {
-      const unsigned long long o = atomicAdd(c.out.overflow_cnt, 1ull);
-      if (o < c.out.overflow_cap) c.out.overflow[o] = q;
-    }
+    if (process_query<TM>(c, ws, cap, q, last_resort != 0) == kOverflow) pass_on<TM>(c, q);
   }
-  stats_flush(ws, c.out.ctr);
+  stats_flush<TM>(ws, c.out.ctr, blockIdx.x);
+}
+
+// Probe records for the strands of one length bucket: thread (i, p) hashes
+// window p of strand list[i] (the bases the tile kernel would stage) and looks
+// up its bucket.  The directory loads of many strands are then in flight at
+// once, and the tile kernel reads one record per probe.
+__global__ __launch_bounds__(256) void k_probe(IndexView ix, Reads rd, const uint32_t *binned,
+                                               const unsigned long long *start,
+                                               const unsigned long long *count, uint32_t stride,
+                                               uint32_t lanes_log2, uint64_t *rec) {
+  const uint32_t *list = binned + *start;
+  const uint64_t n = *count;
+  const uint64_t total = n << lanes_log2;
+  const uint32_t lane_mask = (1u << lanes_log2) - 1;
+  const int T = ix.T, s = ix.s, k = ix.k;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+       g += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t p = (uint32_t)g & lane_mask;
+    if (p >= stride) continue;
+    const uint32_t q = list[g >> lanes_log2];
+    const uint32_t read = q >> 1, strand = q & 1;
+    const uint64_t r0 = rd.off[read];
+    const uint64_t m = rd.off[read + 1] - r0;
+    if (m <= (uint64_t)T) continue;
+    const uint64_t nprobe = (m - T + s - 1) / s;
+    if (p >= nprobe) continue;
+    const uint8_t *b = rd.bytes + r0;
+    const uint64_t w0 = (uint64_t)p * s;
+    uint32_t h = 2166136261u;  // Hashmap.cpp:3197-3211
+    if (strand == 0) {
+      for (int x = 0; x < k; x++) h = (h * 16777619u) ^ upper(b[w0 + x]);
+    } else {
+      for (int x = 0; x < k; x++) h = (h * 16777619u) ^ kComp[upper(b[m - 1 - (w0 + x)])];
+    }
+    h = ((h >> kHashBits) ^ h) & kHashMask;
+    uint64_t r = 0;  // the empty bucket: present, no entries
+    if (h != ix.empty_hash) {
+      const uint64_t lo = ix.dir[h], hi = ix.dir[h + 1];
+      r = hi > lo ? (lo << 24) | (uint64_t)min(hi - lo, (uint64_t)kRecCntMax) : kRecAbsent;
+    }
+    rec[(uint64_t)q * stride + p] = r;
+  }
 }
 
 // Strand merge (Analyser.cpp:592-611): the reverse strand wins only when its
@@ -1261,7 +1428,7 @@ __global__ __launch_bounds__(64) void k_score_seeds(IndexView ix, Luts lut, cons
         lq = sd[3];
       };
       if (s1 > s0) {
-        score_ref<64>(ws.q, m, ix.bytes + ro, ref_len, (uint32_t)(s1 - s0), ix.T,
+        score_ref<BlockTeam<64>>(ws.q, m, ix.bytes + ro, ref_len, (uint32_t)(s1 - s0), ix.T,
                       (flags & MH_CLASSIFY_ULTRAFAST) != 0, ws, seed, sum_r, sum_c, hits);
         score = matrix_score(sum_r, sum_c, hits, m);
         if (score > -1.0f && !rel_score(lut, m, ref_len, score, rel)) rel = -1.0f;
@@ -1294,44 +1461,60 @@ hipError_t ensure_comp_table(int device) {
 
 namespace {
 
-template <int BS>
-hipError_t launch_lds(const Ctx &c, const Caps &cap, const uint32_t *qlist,
-                      const unsigned long long *qstart, uint32_t nq,
-                      const unsigned long long *nq_dev, unsigned long long *head, uint32_t grid,
-                      hipStream_t st) {
-  const size_t lds = ws_bytes(cap);
-  hipFuncSetAttribute((const void *)k_classify<BS>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                      (int)lds);
-  hipLaunchKernelGGL((k_classify<BS>), dim3(grid), dim3(BS), lds, st, c, cap, qlist, qstart, nq,
-                     nq_dev, head);
-  return hipGetLastError();
-}
+// Teams per workgroup for a tile's block field: 16 and 32 are lane teams of
+// a one-wavefront workgroup; 64..1024 are whole workgroups.
+uint32_t teams_of(uint32_t block) { return block < 64 ? 64 / block : 1; }
+uint32_t threads_of(uint32_t block) { return block < 64 ? 64 : block; }
 
-hipError_t launch_tile(const mh_tile &t, const Ctx &c, const Caps &cap, const uint32_t *qlist,
+template <class TM>
+hipError_t launch_team(const Ctx &c, const Caps &cap, const uint32_t *qlist,
                        const unsigned long long *qstart, uint32_t nq,
                        const unsigned long long *nq_dev, unsigned long long *head, uint32_t grid,
                        hipStream_t st) {
-  switch (t.block) {
-    case 64: return launch_lds<64>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
-    case 128: return launch_lds<128>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
-    case 256: return launch_lds<256>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
-    case 512: return launch_lds<512>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
-    default: return launch_lds<1024>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
+  const size_t lds = TM::kPerWave * ws_stride_lds(cap);
+  hipFuncSetAttribute((const void *)k_classify<TM>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                      (int)lds);
+  if (!head) grid = (nq + TM::kPerWave - 1) / TM::kPerWave;  // plain: a query per team
+  hipLaunchKernelGGL((k_classify<TM>), dim3(grid), dim3(TM::kSize * TM::kPerWave), lds, st, c,
+                     cap, qlist, qstart, nq, nq_dev, head);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile(uint32_t block, const Ctx &c, const Caps &cap, const uint32_t *qlist,
+                       const unsigned long long *qstart, uint32_t nq,
+                       const unsigned long long *nq_dev, unsigned long long *head, uint32_t grid,
+                       hipStream_t st) {
+  switch (block) {
+    case 16: return launch_team<LaneTeam<16>>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
+    case 32: return launch_team<LaneTeam<32>>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
+    case 64: return launch_team<BlockTeam<64>>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
+    case 128: return launch_team<BlockTeam<128>>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
+    case 256: return launch_team<BlockTeam<256>>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
+    case 512: return launch_team<BlockTeam<512>>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
+    default:
+      return launch_team<BlockTeam<1024>>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
   }
 }
 
-// Workgroups of one LDS kernel resident per CU at this LDS size.
-uint32_t resident_blocks(uint32_t block, size_t lds) {
+template <class TM>
+uint32_t resident(size_t lds) {
   int n = 0;
-  hipError_t e = hipErrorInvalidValue;
-  switch (block) {
-    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_classify<64>, 64, lds); break;
-    case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_classify<128>, 128, lds); break;
-    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_classify<256>, 256, lds); break;
-    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_classify<512>, 512, lds); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_classify<1024>, 1024, lds);
-  }
+  const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &n, k_classify<TM>, TM::kSize * TM::kPerWave, lds);
   return (e == hipSuccess && n > 0) ? (uint32_t)n : 1u;
+}
+
+// Workgroups of one tile kernel resident per CU with this LDS per workgroup.
+uint32_t resident_blocks(uint32_t block, size_t lds) {
+  switch (block) {
+    case 16: return resident<LaneTeam<16>>(lds);
+    case 32: return resident<LaneTeam<32>>(lds);
+    case 64: return resident<BlockTeam<64>>(lds);
+    case 128: return resident<BlockTeam<128>>(lds);
+    case 256: return resident<BlockTeam<256>>(lds);
+    case 512: return resident<BlockTeam<512>>(lds);
+    default: return resident<BlockTeam<1024>>(lds);
+  }
 }
 
 __global__ void k_max_len(Reads rd, unsigned long long *out) {
@@ -1367,6 +1550,15 @@ constexpr uint32_t kHbmEcap = 1u << 15;
 constexpr uint32_t kBigBlocks = 32;
 constexpr uint32_t kBigEcap = 1u << 19;
 
+// Capacities of a tile's team workspace: the workgroup's LDS is shared by
+// its teams.
+Caps tile_caps(const mh_tile &t, int s, size_t lds_per_block) {
+  return caps_for(t, s, lds_per_block / teams_of(t.block));
+}
+
+// LDS of one workgroup of a tile.
+size_t tile_lds(const mh_tile &t, const Caps &c) { return teams_of(t.block) * ws_stride_lds(c); }
+
 Caps hbm_caps(uint32_t max_read_len, int s, uint32_t ecap) {
   Caps c;
   c.qcap = (max_read_len + 63) & ~63u;
@@ -1380,8 +1572,11 @@ size_t ws_stride(const Caps &c) { return (ws_bytes(c) + 255) & ~size_t(255); }
 }  // namespace
 
 std::vector<mh_tile> default_tiles(uint32_t max_read_len) {
-  const mh_tile base[4] = {{320, 64, 64, 0}, {1024, 128, 512, 0}, {4096, 256, 2048, 0},
-                           {0xFFFFFFFFu, 256, 2048, 0}};
+  // Entry capacities follow the expected bucket entries of each length
+  // (about two per probe on a 5000-reference index at k = 16); short reads
+  // run 2 or 4 to a wavefront.
+  const mh_tile base[6] = {{200, 16, 32, 0},   {400, 32, 64, 0},    {1024, 128, 512, 0},
+                           {2048, 128, 512, 0}, {4096, 128, 1024, 0}, {0xFFFFFFFFu, 1024, 2048, 0}};
   std::vector<mh_tile> t;
   for (mh_tile b : base) {
     if (b.max_len >= max_read_len) {
@@ -1394,16 +1589,32 @@ std::vector<mh_tile> default_tiles(uint32_t max_read_len) {
   return t;
 }
 
+std::vector<mh_tile> active_tiles(const mh_handle *h) {
+  return h->tiles.empty() ? default_tiles(h->max_read_len) : h->tiles;
+}
+
+// Probe records per strand of the first length tile (0: that tile hashes
+// and looks up itself): with MH_CLASSIFY_PREPROBE, one per probe the tile
+// can hold.
+uint32_t probe_stride(const std::vector<mh_tile> &tiles, int s, size_t lds_limit,
+                      uint32_t flags) {
+  if (!(flags & MH_CLASSIFY_PREPROBE)) return 0;
+  const Caps c0 = tile_caps(tiles[0], s, lds_limit);
+  return c0.pcap <= 64 ? c0.pcap : 0u;
+}
+
 // Scratch layout for one classify call (bytes from the handle's arena):
 //   QueryOut[2n] | overflow lists u32[2n] x 2 | binned list u32[2n] |
 //   merge counts u32[n+1] | merge offsets u32[n+1] | tie pool u32[f*2n] |
-//   bucket words u64[64] | scan temp | large-kernel workspaces
+//   bucket words u64[64] | scan temp | probe records u64[2n * stride] |
+//   HBM-tier workspaces
 struct Layout {
-  size_t qo, overflow, overflow2, binned, cnt, off, ties, bw, scan, big, total;
+  size_t qo, overflow, overflow2, binned, cnt, off, ties, bw, scan, rec, big, total;
   size_t scan_bytes, tie_cap;
 };
 
-Layout layout(uint32_t n_reads, uint32_t max_read_len, int s, uint32_t tie_factor) {
+Layout layout(uint32_t n_reads, uint32_t max_read_len, int s, uint32_t tie_factor,
+              uint32_t rec_stride) {
   const size_t nq = 2ull * n_reads;
   Layout L;
   size_t o = 0;
@@ -1425,26 +1636,27 @@ Layout layout(uint32_t n_reads, uint32_t max_read_len, int s, uint32_t tie_facto
   hipcub::DeviceScan::ExclusiveSum(nullptr, L.scan_bytes, (const uint32_t *)nullptr,
                                    (uint32_t *)nullptr, (int)(n_reads + 1));
   L.scan = take(L.scan_bytes);
+  L.rec = take(nq * rec_stride * 8ull);
   L.big = take(std::max(kHbmBlocks * ws_stride(hbm_caps(max_read_len, s, kHbmEcap)),
                         kBigBlocks * ws_stride(hbm_caps(max_read_len, s, kBigEcap))));
   L.total = o;
   return L;
 }
 
-size_t classify_scratch_bytes(uint32_t n_reads, uint32_t max_read_len, int s) {
-  return layout(n_reads, max_read_len, s, 4).total;
-}
-
-size_t classify_scratch_bytes_f(uint32_t n_reads, uint32_t max_read_len, int s,
-                                uint32_t tie_factor) {
-  return layout(n_reads, max_read_len, s, tie_factor).total;
+size_t classify_scratch_bytes(const mh_handle *h, uint32_t n_reads, int s, uint32_t tie_factor,
+                              uint32_t flags) {
+  const uint32_t stride = probe_stride(active_tiles(h), s, h->lds_per_block, flags);
+  return layout(n_reads, h->max_read_len, s, tie_factor, stride).total;
 }
 
 mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool batched) {
   hipError_t e = hipSuccess;
   const uint32_t n = a.rd.n;
   const uint64_t nq = 2ull * n;
-  const Layout L = layout(n, h->max_read_len, a.ix.s, x.tie_factor);
+  const std::vector<mh_tile> tiles = active_tiles(h);
+  const uint32_t rec_stride =
+      batched ? probe_stride(tiles, a.ix.s, h->lds_per_block, a.flags) : 0u;
+  const Layout L = layout(n, h->max_read_len, a.ix.s, x.tie_factor, rec_stride);
   mh_status st = ensure_scratch(h, x, L.total);
   if (st) return st;
   unsigned char *base = (unsigned char *)x.d_scratch;
@@ -1465,11 +1677,13 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
   c.rd = a.rd;
   c.flags = a.flags;
   c.max_len = h->max_read_len;
+  c.rec = nullptr;
+  c.rec_stride = 0;
   c.out = a.out;
   c.out.q = qo;
   c.out.ctr = x.d_ctr;
   c.out.overflow = overflow;
-  c.out.overflow_cnt = &x.d_ctr->overflow_n;
+  c.out.overflow_cnt = &x.d_ctr->tier_n[0];
   c.out.overflow_cap = (uint32_t)nq;
   c.out.ties = (uint32_t *)(base + L.ties);
   c.out.tie_cap = L.tie_cap;
@@ -1482,7 +1696,6 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
   hipLaunchKernelGGL(k_zero, dim3(8), dim3(256), 0, s, (unsigned long long *)x.d_ctr, zn);
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, bw, 64u);
   if (nq) {
-    std::vector<mh_tile> tiles = h->tiles.empty() ? default_tiles(h->max_read_len) : h->tiles;
     const unsigned gr = (unsigned)((n + 255) / 256);
     if (!batched) {
       // Plain form: one launch per batch, one workgroup per query, with the
@@ -1495,11 +1708,12 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
       if (e != hipSuccess) return hip_error(h, e, "max read length");
       size_t ti = 0;
       while (ti + 1 < tiles.size() && mlen > tiles[ti].max_len) ti++;
-      const mh_tile &t0 = tiles[ti];
-      const Caps cap = caps_for(t0, a.ix.s, h->lds_per_block);
+      // whole-workgroup teams here: the plain form is the simple one
+      mh_tile t0 = tiles[ti];
+      t0.block = std::max<uint32_t>(t0.block, 64);
+      const Caps cap = tile_caps(t0, a.ix.s, h->lds_per_block);
       if (timed) hipEventRecord(x.ev[9], s);
-      e = launch_tile(t0, c, cap, nullptr, nullptr, (uint32_t)nq, nullptr, nullptr, (uint32_t)nq,
-                      s);
+      e = launch_tile(t0.block, c, cap, nullptr, nullptr, (uint32_t)nq, nullptr, nullptr, 0, s);
       if (e != hipSuccess) return hip_error(h, e, "k_classify");
       if (timed) {
         hipEventRecord(x.ev[1], s);
@@ -1515,16 +1729,30 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
       hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, bw, nb, bw + 16);
       hipLaunchKernelGGL(k_bin_scatter, dim3(gr), dim3(256), 0, s, rd, d_maxlen, nb, bw + 16,
                          bw + 8, binned);
+      // Probe records for the first tile's strands.
+      Ctx c0 = c;
+      if (rec_stride) {
+        uint64_t *rec = (uint64_t *)(base + L.rec);
+        uint32_t lg = 0;
+        while ((1u << lg) < rec_stride) lg++;
+        const uint64_t threads = nq << lg;
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>((threads + 255) / 256, 1u << 20);
+        hipLaunchKernelGGL(k_probe, dim3(blocks), dim3(256), 0, s, a.ix, rd,
+                           (const uint32_t *)binned, (const unsigned long long *)(bw + 16),
+                           (const unsigned long long *)bw, rec_stride, lg, rec);
+        c0.rec = rec;
+        c0.rec_stride = rec_stride;
+      }
       // Long buckets first so that their tail overlaps with the short ones.
       if (timed) hipEventRecord(x.ev[9], s);
       for (int b = (int)nb - 1; b >= 0; b--) {
         const mh_tile &t = tiles[b];
-        const Caps cap = caps_for(t, a.ix.s, h->lds_per_block);
+        const Caps cap = tile_caps(t, a.ix.s, h->lds_per_block);
         uint32_t grid = t.grid;
-        if (!grid) grid = h->cu_count * resident_blocks(t.block, ws_bytes(cap));
+        if (!grid) grid = h->cu_count * resident_blocks(t.block, tile_lds(t, cap));
         // queue b: binned[start[b], start[b] + cnt[b]), head queue_head[b]
-        e = launch_tile(t, c, cap, binned, bw + 16 + b, 0, bw + b, &x.d_ctr->queue_head[b],
-                        grid, s);
+        e = launch_tile(t.block, b == 0 ? c0 : c, cap, binned, bw + 16 + b, 0, bw + b,
+                        &x.d_ctr->queue_head[b], grid, s);
         if (e != hipSuccess) return hip_error(h, e, "k_classify batched");
         if (timed) {
           hipEventRecord(x.ev[1 + x.timed_buckets], s);
@@ -1532,35 +1760,40 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
         }
       }
     }
-    // Middle tier: queries that overflowed their tile, in a 256-thread LDS
-    // kernel with the largest workspace that fits; what still overflows goes
-    // to the HBM workspace.
-    {
-      const mh_tile mid{h->max_read_len, 256, 4096, 0};
-      const Caps mcap = caps_for(mid, a.ix.s, h->lds_per_block);
-      Ctx cm = c;
-      cm.out.overflow = overflow2;
-      cm.out.overflow_cnt = &x.d_ctr->overflow2_n;
-      const uint32_t mgrid = h->cu_count * resident_blocks(256, ws_bytes(mcap));
-      e = launch_tile(mid, cm, mcap, overflow, nullptr, 0, &x.d_ctr->overflow_n,
-                      &x.d_ctr->mid_head, mgrid, s);
-      if (e != hipSuccess) return hip_error(h, e, "k_classify middle tier");
+    // Overflow tiers, each a persistent kernel over the list the one before
+    // it filled (two lists in turn):
+    //   0  second chance: 128-thread teams, 1024 bases, 512 entries
+    //   1  middle: 256-thread teams with the largest LDS workspace
+    //   2  HBM workspaces of 2^15 entries over kHbmBlocks workgroups
+    //   3  last resort: HBM workspaces of 2^19 entries, flags what is left
+    uint32_t *list[2] = {overflow, overflow2};
+    DevCounters *d = x.d_ctr;
+    const mh_tile lds_tiers[2] = {
+        {std::min<uint32_t>(1024u, h->max_read_len), 128, 512, 0},
+        {h->max_read_len, 256, 4096, 0}};
+    for (int t = 0; t < 2; t++) {
+      const Caps tc = tile_caps(lds_tiers[t], a.ix.s, h->lds_per_block);
+      Ctx ct = c;
+      ct.out.overflow = list[(t + 1) & 1];
+      ct.out.overflow_cnt = &d->tier_n[t + 1];
+      const uint32_t grid =
+          h->cu_count * resident_blocks(lds_tiers[t].block, tile_lds(lds_tiers[t], tc));
+      e = launch_tile(lds_tiers[t].block, ct, tc, list[t & 1], nullptr, 0, &d->tier_n[t],
+                      &d->tier_head[t], grid, s);
+      if (e != hipSuccess) return hip_error(h, e, "k_classify overflow tier");
     }
     {
-      // First HBM tier, then the last resort; the tile overflow list is free
-      // again and carries what the first tier passes on.
       const Caps ca = hbm_caps(h->max_read_len, a.ix.s, kHbmEcap);
       Ctx ch = c;
-      ch.out.overflow = overflow;
-      ch.out.overflow_cnt = &x.d_ctr->overflow3_n;
+      ch.out.overflow = list[1];
+      ch.out.overflow_cnt = &d->tier_n[3];
       hipLaunchKernelGGL(k_classify_hbm<256>, dim3(kHbmBlocks), dim3(256), 0, s, ch, ca, big_ws,
-                         ws_stride(ca), (const uint32_t *)overflow2,
-                         (const unsigned long long *)&x.d_ctr->overflow2_n,
-                         &x.d_ctr->queue_head[8], 0);
+                         ws_stride(ca), (const uint32_t *)list[0],
+                         (const unsigned long long *)&d->tier_n[2], &d->tier_head[2], 0);
       const Caps cb = hbm_caps(h->max_read_len, a.ix.s, kBigEcap);
       hipLaunchKernelGGL(k_classify_hbm<1024>, dim3(kBigBlocks), dim3(1024), 0, s, c, cb, big_ws,
-                         ws_stride(cb), (const uint32_t *)overflow,
-                         (const unsigned long long *)&x.d_ctr->overflow3_n, &x.d_ctr->big_head, 1);
+                         ws_stride(cb), (const uint32_t *)list[1],
+                         (const unsigned long long *)&d->tier_n[3], &d->tier_head[3], 1);
     }
     if (timed) hipEventRecord(x.ev[10], s);
     hipLaunchKernelGGL(k_merge_count, dim3(std::min<uint32_t>(gr, 4096)), dim3(256), 0, s,
@@ -1595,7 +1828,7 @@ void counters_from(const DevCounters &d, uint64_t n_reads, mh_counters &o) {
   o.seeds = t[kSeeds];
   o.cands = t[kCands];
   o.direct = t[kDirect];
-  o.overflow = d.overflow_n + d.overflow2_n + d.overflow3_n;
+  o.overflow = d.tier_n[0] + d.tier_n[1] + d.tier_n[2] + d.tier_n[3];
   o.ext_compares = t[kExt];
 }
 

@@ -64,15 +64,13 @@ constexpr int kStatSlots = 256;
 
 // Device counters, zeroed before each classify call.
 struct DevCounters {
-  unsigned long long top_used;    // final top-tie pool fill (set by the merge)
+  unsigned long long top_used;      // final top-tie pool fill (set by the merge)
   unsigned long long cand_used;
-  unsigned long long overflow_n;  // queries the length tiles pass to the middle tier
-  unsigned long long big_head;    // queue head of the last-resort HBM tier
-  unsigned long long tie_used;    // scratch tie pool fill
-  unsigned long long overflow2_n; // queries the middle tier passes to the HBM tiers
-  unsigned long long mid_head;    // middle-tier queue head
-  unsigned long long overflow3_n; // queries the first HBM tier passes to the last resort
-  unsigned long long queue_head[16];
+  unsigned long long tie_used;      // scratch tie pool fill
+  unsigned long long pad;
+  unsigned long long tier_n[4];     // queries passed on to overflow tier t
+  unsigned long long tier_head[4];  // queue heads of the overflow tiers
+  unsigned long long queue_head[8]; // queue heads of the length buckets
   unsigned long long stats[kStatSlots][8];
 };
 
@@ -204,9 +202,8 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
 // Complement table in constant memory (once per device, before any capture).
 hipError_t ensure_comp_table(int device);
 // Scratch bytes one classify call needs.
-size_t classify_scratch_bytes(uint32_t n_reads, uint32_t max_read_len, int s);
-size_t classify_scratch_bytes_f(uint32_t n_reads, uint32_t max_read_len, int s,
-                                uint32_t tie_factor);
+size_t classify_scratch_bytes(const mh_handle *h, uint32_t n_reads, int s, uint32_t tie_factor,
+                              uint32_t flags);
 // Reads the counters of the last call on x (x's stream must be idle).
 void counters_from(const DevCounters &d, uint64_t n_reads, mh_counters &out);
 mh_status launch_score_seeds(mh_handle *h, const IndexView &ix, const Luts &luts,

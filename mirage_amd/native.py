@@ -27,6 +27,7 @@ CAND_DTYPE = np.dtype([("read", "<u4"), ("strand", "<u4"), ("ref", "<u4"), ("hit
 COUNTER_FIELDS = ("queries", "probes", "hits", "entries", "seeds", "cands", "direct", "overflow",
                   "ext_compares")
 MH_CLASSIFY_TIMING = 8
+MH_CLASSIFY_PREPROBE = 32
 MH_READ_BADBASE = 1
 MH_READ_TOO_LONG = 2
 MH_READ_CAPACITY = 4

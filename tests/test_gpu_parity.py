@@ -90,7 +90,8 @@ def workloads():
 
 
 @pytest.mark.parametrize("num,scale", WORKLOADS)
-@pytest.mark.parametrize("flags", [0, native.MH_CLASSIFY_BATCHED])
+@pytest.mark.parametrize("flags", [0, native.MH_CLASSIFY_BATCHED,
+                                   native.MH_CLASSIFY_BATCHED | native.MH_CLASSIFY_PREPROBE])
 def test_workload_matches_oracle(handle, workloads, num, scale, flags):
     w, (rb, ro, qb, qo, truth) = workloads(num, scale)
     ix = handle.build_index(rb, ro, w.threshold)

@@ -72,12 +72,21 @@ def test_repeat_reads_reach_every_tier(handle, flags):
     assert max(len(t) for t in native.tie_sets(out, top)) == 300
 
 
+TILE_TABLES = {
+    "block64": [(320, 64, 64, 0), (None, 64, 64, 0)],
+    "lanes": [(200, 16, 32, 0), (600, 32, 32, 0), (None, 64, 64, 0)],
+}
+
+
 @pytest.mark.parametrize("flags", [0, native.MH_CLASSIFY_BATCHED])
-def test_tiny_tiles_send_work_to_the_tiers(handle, flags):
-    """With 64-entry tiles most strands overflow; results are unchanged."""
+@pytest.mark.parametrize("table", sorted(TILE_TABLES))
+def test_tiny_tiles_send_work_to_the_tiers(handle, flags, table):
+    """With small tiles (whole-workgroup teams, or lane teams of 16 and 32
+    lanes) many strands overflow; results are unchanged."""
     saved = handle.get_tiles()
     try:
-        handle.set_tiles([(320, 64, 64, 0), (handle.max_read_len, 64, 64, 0)])
+        handle.set_tiles([(ml or handle.max_read_len, b, e, g)
+                          for ml, b, e, g in TILE_TABLES[table]])
         w = D.workload(3, 0.0005)
         w.refs.n = 300
         rb, ro, qb, qo, _ = D.generate(w)
