@@ -205,7 +205,7 @@ mh_status mh_score_seeds(mh_handle *h, const mh_index *idx, const uint8_t *d_que
 /* Tile choice for the length-bucketed persistent form. */
 typedef struct {
   uint32_t max_len;        /* bucket holds reads with length <= max_len */
-  uint32_t block;          /* threads per query: 16 or 32 (lane teams, 64 / block
+  uint32_t block;          /* threads per query: 8, 16 or 32 (lane teams, 64 / block
                               queries per one-wavefront workgroup) or 64..1024
                               (a workgroup per query), a power of two */
   uint32_t entries_cap;    /* LDS capacity in bucket entries (power of two, >= 32) */

@@ -108,7 +108,8 @@ bool valid_tiles(const mh_tile *t, uint32_t n, uint32_t max_len) {
   if (n == 0 || n > 8) return false;
   for (uint32_t i = 0; i < n; i++) {
     const uint32_t b = t[i].block;
-    if (b != 16 && b != 32 && b != 64 && b != 128 && b != 256 && b != 512 && b != 1024)
+    if (b != 8 && b != 16 && b != 32 && b != 64 && b != 128 && b != 256 && b != 512 &&
+        b != 1024)
       return false;
     if (t[i].entries_cap < 32 || (t[i].entries_cap & (t[i].entries_cap - 1))) return false;
     if (i && t[i].max_len <= t[i - 1].max_len) return false;

@@ -1185,6 +1185,7 @@ __global__ __launch_bounds__(TM::kSize * TM::kPerWave) void k_classify(
       : max(1ull, n / ((unsigned long long)gridDim.x * G * 4ull));
   const uint32_t chunk = (uint32_t)per_team * G;
   for (;;) {
+    if (n == 0) break;  // an empty bucket or tier
     __syncthreads();
     if (threadIdx.x == 0) {
       const unsigned long long i = atomicAdd(head, (unsigned long long)chunk);
@@ -1219,7 +1220,7 @@ __global__ __launch_bounds__(BS) void k_classify_hbm(Ctx c, Caps cap, unsigned c
     __syncthreads();
     if (threadIdx.x == 0) {
       const unsigned long long n = *count;
-      const unsigned long long i = atomicAdd(head, 1ull);
+      const unsigned long long i = n ? atomicAdd(head, 1ull) : 0ull;
       slot = (i < n && i < c.out.overflow_cap) ? (uint32_t)i : 0xFFFFFFFFu;
     }
     __syncthreads();
@@ -1461,8 +1462,8 @@ hipError_t ensure_comp_table(int device) {
 
 namespace {
 
-// Teams per workgroup for a tile's block field: 16 and 32 are lane teams of
-// a one-wavefront workgroup; 64..1024 are whole workgroups.
+// Teams per workgroup for a tile's block field: 8, 16 and 32 are lane teams
+// of a one-wavefront workgroup; 64..1024 are whole workgroups.
 uint32_t teams_of(uint32_t block) { return block < 64 ? 64 / block : 1; }
 uint32_t threads_of(uint32_t block) { return block < 64 ? 64 : block; }
 
@@ -1485,6 +1486,7 @@ hipError_t launch_tile(uint32_t block, const Ctx &c, const Caps &cap, const uint
                        const unsigned long long *nq_dev, unsigned long long *head, uint32_t grid,
                        hipStream_t st) {
   switch (block) {
+    case 8: return launch_team<LaneTeam<8>>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
     case 16: return launch_team<LaneTeam<16>>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
     case 32: return launch_team<LaneTeam<32>>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
     case 64: return launch_team<BlockTeam<64>>(c, cap, qlist, qstart, nq, nq_dev, head, grid, st);
@@ -1507,6 +1509,7 @@ uint32_t resident(size_t lds) {
 // Workgroups of one tile kernel resident per CU with this LDS per workgroup.
 uint32_t resident_blocks(uint32_t block, size_t lds) {
   switch (block) {
+    case 8: return resident<LaneTeam<8>>(lds);
     case 16: return resident<LaneTeam<16>>(lds);
     case 32: return resident<LaneTeam<32>>(lds);
     case 64: return resident<BlockTeam<64>>(lds);
