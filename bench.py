@@ -185,6 +185,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-scale", type=float, default=1.0)
     ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also copy 1 GiB device to device once, the reference kernel for "
+                         "scaling HBM byte counters in a profiler run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the classification path on a "
@@ -225,6 +228,14 @@ def main():
         f"same results {graph_same}")
     batched_rps = n / min(sec, gsec)
     sec = min(sec, gsec)
+    if args.calibrate_copy:
+        a = torch.ones(1 << 30, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB read + 1 GiB written")
+        del a, b
 
     plain = None
     if not args.no_plain:
@@ -265,11 +276,21 @@ def main():
     alg_bytes = (16 * ctr["probes"] + 8 * ctr["entries"] + 2 * ctr["ext_compares"] +
                  2 * int(qo[-1]) + 32 * ctr["queries"])
     achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
+    kernel_name = "k_classify<LaneTeam<16>> (reads <= 200 bp)"
+    # HBM bytes per launch of this kernel from the profiler's FETCH_SIZE and
+    # WRITE_SIZE counters, scaled by what they read for a 1 GiB copy (the
+    # summary in profiles/ says how it was measured).
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", "k_classify_lane16_workload2_traffic.json")
+    if os.path.exists(tpath) and args.scale == 1.0:
+        with open(tpath) as f:
+            tj = json.load(f)
+        if tj.get("kernel") == kernel_name:
+            traffic, traffic_src = int(tj["traffic_bytes"]), tj.get("source")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_classify<LaneTeam<16>> (reads <= 200 bp)",
-                "kernel_ms": round(kernel_ms, 4),
-                "alg_bytes_per_launch": alg_bytes}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "traffic_source": traffic_src, "kernel": kernel_name,
+                "kernel_ms": round(kernel_ms, 4), "alg_bytes_per_launch": alg_bytes}
 
     serve = None
     if not args.no_serve:
