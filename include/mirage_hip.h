@@ -122,6 +122,7 @@ typedef struct {
   float big_ms;            /* middle tier and HBM-workspace tiers */
   float merge_ms;          /* strand merge */
   uint32_t n_buckets;
+  float probe_ms;          /* probe pre-pass (MH_CLASSIFY_PREPROBE), else 0 */
 } mh_timing;
 
 /* ---- handle ------------------------------------------------------------ */

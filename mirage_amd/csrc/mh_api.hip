@@ -428,6 +428,7 @@ mh_status mh_get_timing(mh_handle *h, mh_timing *out) {
     prev = 1 + (int)i;
   }
   out->big_ms = ms(prev, 10);
+  if (x.timed_probe) out->probe_ms = ms(12, 9);
   out->merge_ms = ms(10, 11);
   return MH_OK;
 }

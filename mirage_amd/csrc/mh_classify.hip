@@ -336,16 +336,20 @@ struct Ctx {
   Reads rd;
   uint32_t flags;
   uint32_t max_len;  // longest read the handle accepts
-  // Probe records written by k_probe for this launch's strands (rec[q *
-  // rec_stride + p]), or null: the kernel hashes and looks up itself.
+  // Probe records written by k_probe for this launch's strands of length
+  // up to rec_maxlen, or null (the kernel stages, checks, hashes and looks
+  // up itself): rec[q * rec_stride] is kRecBad for a strand with a base
+  // outside the alphabet, else 0; rec[q * rec_stride + 1 + p] is probe p.
   const uint64_t *rec;
   uint32_t rec_stride;
+  uint32_t rec_maxlen;
   Outputs out;
 };
 
 // Probe record: lo << 24 | cnt of the probe's bucket; kRecAbsent when the
 // hash has no bucket; cnt kRecCntMax when the count does not fit 24 bits.
 constexpr uint64_t kRecAbsent = ~0ull;
+constexpr uint64_t kRecBad = ~0ull - 1;
 constexpr uint32_t kRecCntMax = 0xFFFFFFu;
 
 enum : uint32_t { kDone = 0, kOverflow = 1 };
@@ -689,6 +693,234 @@ __device__ void filter_sorted(const Ws &ws, uint32_t E, uint32_t &nP) {
   TM::sync();
 }
 
+enum : uint32_t { kSmallNo = 0, kSmallDirect, kSmallSorted };
+
+// Compare-exchange partner values for the register sort: (key, v) ordered
+// by key, then v.
+__device__ __forceinline__ bool elem_gt(uint64_t ka, uint32_t va, uint64_t kb, uint32_t vb) {
+  return ka > kb || (ka == kb && va > vb);
+}
+
+template <class TM>
+__device__ __forceinline__ uint64_t team_or64(uint64_t x) {
+  constexpr int W = TM::kWorkerLanes;
+#pragma unroll
+  for (int d = W / 2; d > 0; d >>= 1) {
+    const uint32_t lo = __shfl_xor((uint32_t)x, d, W), hi = __shfl_xor((uint32_t)(x >> 32), d, W);
+    x |= ((uint64_t)hi << 32) | lo;
+  }
+  return x;
+}
+
+// Small working sets (at most 64 entries and 64 hits, a team inside one
+// wavefront) in registers: element i of the team's array is slot i & 1 of
+// lane i >> 1.  Gives the decisions of the LDS path below.
+//  * Direct hit: tested on the gathered entries with ballots (r* from the
+//    first bucket: as many entries as hits, all on its diagonal; no other
+//    reference of the second bucket present in every bucket 1..N-2).
+//  * Otherwise the entries are sorted by (ref, refPos, qPos) with a bitonic
+//    network across the lanes; a segmented OR-scan over each reference's
+//    run of elements gives its hit mask (bit i: an entry at hit index i),
+//    whose longest run of set bits is the reference's longest run of
+//    consecutive hit indices (repeats of one hit set one bit).  Kept pairs
+//    (run >= cut) are compacted in sorted order into k2/v2.
+// Results: kSmallDirect (P, ranks and chains set up as the direct path
+// does), kSmallSorted (kept pairs sorted in k2/v2, nP set).
+template <class TM>
+__device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t nhits,
+                               uint32_t nprobe, uint32_t &nP) {
+  constexpr uint32_t W = TM::kWorkerLanes;
+  static_assert(TM::kSize == W, "one-wavefront teams");
+  const uint32_t lane = TM::rank();
+  uint64_t key[2];
+  uint32_t ref[2], pos[2], qp[2], hit[2];
+  bool ok[2];
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const uint32_t e = 2 * lane + j;
+    ok[j] = e < E;
+    ref[j] = pos[j] = qp[j] = 0xFFFFFFFFu;
+    hit[j] = 63;
+    if (ok[j]) {
+      const uint64_t k = ws.k1[e];
+      ref[j] = (uint32_t)(k >> 32);
+      hit[j] = (uint32_t)k;
+      pos[j] = ws.v1[e];
+      qp[j] = ws.hit_j[hit[j]];
+    }
+    key[j] = ((uint64_t)ref[j] << 32) | pos[j];
+  }
+  if (!(c.flags & MH_CLASSIFY_NO_DIRECT) && nhits == nprobe && nhits >= 3) {
+    const uint32_t N = nhits;
+    const uint32_t c0 = ws.hit_base[1];  // entries of the first bucket
+    uint32_t nstar = 0, rstar = 0;
+    for (uint32_t b = 0; b < c0; b++) {
+      const uint32_t r = (uint32_t)(ws.k1[b] >> 32), p0 = ws.v1[b];  // hit 0: qPos 0
+      uint32_t cm = 0, con = 0;
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const bool m = ok[j] && ref[j] == r;
+        cm += __popcll(TM::wballot(m));
+        con += __popcll(TM::wballot(m && pos[j] - qp[j] == p0));
+      }
+      if (cm == N && con == N) {  // one entry per bucket, all on r's diagonal
+        nstar++;
+        rstar = r;
+      }
+    }
+    bool direct = nstar == 1;
+    if (direct) {
+      const uint64_t full = N == 64 ? ~0ull : ((1ull << N) - 1);
+      const uint64_t mid = full & ~1ull & ~(1ull << (N - 1));
+      for (uint32_t b = ws.hit_base[1]; b < ws.hit_base[2] && direct; b++) {
+        const uint32_t r = (uint32_t)(ws.k1[b] >> 32);
+        if (r == rstar) continue;
+        uint64_t bits = 0;
+#pragma unroll
+        for (int j = 0; j < 2; j++) bits |= (ok[j] && ref[j] == r) ? (1ull << hit[j]) : 0ull;
+        bits = team_or64<TM>(bits);
+        if ((bits & mid) == mid) direct = false;
+      }
+    }
+    if (direct) {
+      TM::sync();
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+        if (ok[j] && ref[j] == rstar) {  // r*'s pairs in hit order, rank 0
+          ws.k2[hit[j]] = (uint64_t)pos[j];
+          ws.v2[hit[j]] = qp[j];
+        }
+      if (lane == 0) {
+        ws.rref[0] = rstar;
+        ws.t3[0] = N;                // chain maximum of rank 0
+        ws.v1[0] = 0;                // chains of rank 0 start at chain 0
+        ((uint32_t *)ws.k1)[0] = 0;  // chain 0 spans pairs [0, N-1]
+        ((uint32_t *)ws.k1)[1] = N - 1;
+        ws.stats[kDirect] += 1;
+      }
+      TM::sync();
+      nP = N;
+      return kSmallDirect;
+    }
+  }
+  // Bitonic sort of next_pow2(E) elements by (key, v); v = qPos << 6 | hit
+  // (qPos orders v, as entries with equal keys differ in qPos).
+  uint32_t v[2];
+#pragma unroll
+  for (int j = 0; j < 2; j++) v[j] = ok[j] ? ((qp[j] << 6) | hit[j]) : 0xFFFFFFFFu;
+  const uint32_t n2 = next_pow2(E < 2 ? 2 : E);
+  for (uint32_t k = 2; k <= n2; k <<= 1) {
+    for (uint32_t d = k >> 1; d > 1; d >>= 1) {
+      const uint32_t lx = d >> 1;  // partner lane = lane ^ lx
+      const bool lower = (lane & lx) == 0;
+      const bool asc = ((2 * lane) & k) == 0;
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const uint32_t plo = __shfl_xor((uint32_t)key[j], lx, W);
+        const uint32_t phi = __shfl_xor((uint32_t)(key[j] >> 32), lx, W);
+        const uint32_t pv = __shfl_xor(v[j], lx, W);
+        const uint64_t pk = ((uint64_t)phi << 32) | plo;
+        const bool pgt = elem_gt(pk, pv, key[j], v[j]);
+        const bool take = (lower == asc) ? !pgt && (pk != key[j] || pv != v[j]) : pgt;
+        if (take) {
+          key[j] = pk;
+          v[j] = pv;
+        }
+      }
+    }
+    // d == 1: the two slots of a lane
+    const bool asc = ((2 * lane) & k) == 0;
+    if (elem_gt(key[0], v[0], key[1], v[1]) == asc) {
+      const uint64_t tk = key[0];
+      const uint32_t tv = v[0];
+      key[0] = key[1];
+      v[0] = v[1];
+      key[1] = tk;
+      v[1] = tv;
+    }
+  }
+  // Segment heads (a new reference), then each reference's hit mask: a
+  // segmented OR over its elements, forward and backward.
+  const uint32_t pk_lo = __shfl_up((uint32_t)key[1], 1, W);
+  const uint32_t pk_hi = __shfl_up((uint32_t)(key[1] >> 32), 1, W);
+  bool head[2], valid[2];
+  uint64_t bit[2];
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    valid[j] = key[j] != ~0ull;
+    bit[j] = valid[j] ? (1ull << (v[j] & 63)) : 0ull;
+  }
+  head[0] = lane == 0 || pk_hi != (uint32_t)(key[0] >> 32);
+  head[1] = (key[0] >> 32) != (key[1] >> 32);
+  // forward inclusive
+  uint64_t f1 = head[1] ? bit[1] : (bit[0] | bit[1]);
+  bool ff = head[0] || head[1];
+  uint64_t fv = f1;
+#pragma unroll
+  for (uint32_t d = 1; d < W; d <<= 1) {
+    const uint32_t lo = __shfl_up((uint32_t)fv, d, W), hi = __shfl_up((uint32_t)(fv >> 32), d, W);
+    const int pf = __shfl_up((int)ff, d, W);
+    if (lane >= d) {
+      if (!ff) fv |= ((uint64_t)hi << 32) | lo;
+      ff = ff || pf;
+    }
+  }
+  const uint32_t xlo = __shfl_up((uint32_t)fv, 1, W), xhi = __shfl_up((uint32_t)(fv >> 32), 1, W);
+  const uint64_t before = lane == 0 ? 0ull : (((uint64_t)xhi << 32) | xlo);
+  const uint64_t inc0 = head[0] ? bit[0] : (before | bit[0]);
+  const uint64_t inc1 = head[1] ? bit[1] : (inc0 | bit[1]);
+  // backward inclusive (a segment ends where the next element is a head)
+  const int nh0 = __shfl_down((int)head[0], 1, W);
+  const bool tail1 = lane == W - 1 || nh0;
+  const bool tail0 = head[1];
+  uint64_t bv = tail0 ? bit[0] : (bit[0] | bit[1]);
+  bool bf = tail0 || tail1;
+#pragma unroll
+  for (uint32_t d = 1; d < W; d <<= 1) {
+    const uint32_t lo = __shfl_down((uint32_t)bv, d, W), hi = __shfl_down((uint32_t)(bv >> 32), d, W);
+    const int pf = __shfl_down((int)bf, d, W);
+    if (lane + d < W) {
+      if (!bf) bv |= ((uint64_t)hi << 32) | lo;
+      bf = bf || pf;
+    }
+  }
+  const uint32_t ylo = __shfl_down((uint32_t)bv, 1, W), yhi = __shfl_down((uint32_t)(bv >> 32), 1, W);
+  const uint64_t after = lane == W - 1 ? 0ull : (((uint64_t)yhi << 32) | ylo);
+  const uint64_t suf1 = tail1 ? bit[1] : (after | bit[1]);
+  const uint64_t suf0 = tail0 ? bit[0] : (suf1 | bit[0]);
+  const uint64_t mask[2] = {inc0 | suf0, inc1 | suf1};
+  // Filter: runs, M, cut, keep.
+  uint32_t run[2], mr = 0;
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    uint64_t x = mask[j];
+    uint32_t r = 0;
+    while (x) {
+      x &= x >> 1;
+      r++;
+    }
+    run[j] = r;
+    mr = max(mr, r);
+  }
+  const uint32_t maxrun = TM::max(mr, ws.red);
+  const uint32_t cut = maxrun >= 2 ? maxrun - 1 : 1;
+  const bool keep0 = valid[0] && run[0] >= cut, keep1 = valid[1] && run[1] >= cut;
+  uint32_t tot;
+  const uint32_t at = TM::excl_scan((keep0 ? 1u : 0u) + (keep1 ? 1u : 0u), ws.red, tot);
+  TM::sync();
+  if (keep0) {
+    ws.k2[at] = key[0];
+    ws.v2[at] = v[0] >> 6;
+  }
+  if (keep1) {
+    ws.k2[at + (keep0 ? 1 : 0)] = key[1];
+    ws.v2[at + (keep0 ? 1 : 0)] = v[1] >> 6;
+  }
+  nP = tot;
+  TM::sync();
+  return kSmallSorted;
+}
+
 // Processes query q.  Returns kOverflow (uniformly) when the working set
 // exceeds the workspace capacities; nothing is written in that case.
 template <class TM>
@@ -723,15 +955,27 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
   const uint32_t m = (uint32_t)m64;
 
   // Stage the strand: uppercase, reverse complement for strand 1
-  // (Sequences.cpp:43-52, :795-841).
-  int bad = 0;
-  for (uint32_t x = tid; x < m; x += BS) {
-    uint8_t ch = upper(c.rd.bytes[r0 + (strand ? (m - 1 - x) : x)]);
-    const uint8_t cc = kComp[ch];  // zero outside the nucleotide alphabet
-    bad |= (cc == 0);
-    ws.q[x] = strand ? cc : ch;
+  // (Sequences.cpp:43-52, :795-841).  With probe records the base check is
+  // done and staging waits until a seed needs the bases.
+  auto stage = [&]() {
+    int b = 0;
+    for (uint32_t x = tid; x < m; x += BS) {
+      uint8_t ch = upper(c.rd.bytes[r0 + (strand ? (m - 1 - x) : x)]);
+      const uint8_t cc = kComp[ch];  // zero outside the nucleotide alphabet
+      b |= (cc == 0);
+      ws.q[x] = strand ? cc : ch;
+    }
+    return TM::any(b);
+  };
+  const bool use_rec = c.rec != nullptr && m <= c.rec_maxlen;
+  const uint64_t *rec = use_rec ? c.rec + (uint64_t)q * c.rec_stride : nullptr;
+  bool staged = false, bad;
+  if (use_rec) {
+    bad = rec[0] == kRecBad;
+  } else {
+    bad = stage();
+    staged = true;
   }
-  bad = TM::any(bad);
   if (bad) {
     res.status = MH_READ_BADBASE;
     write_query<TM>(c, q, res);
@@ -751,7 +995,6 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
 
   // Probe every s-th window; compact present hashes into the hit list.
   // With probe records, the lookups were made beforehand by k_probe.
-  const bool use_rec = c.rec && nprobe <= c.rec_stride;
   uint32_t nhits = 0, big = 0;
   for (uint32_t base = 0; base < nprobe; base += BS) {
     const uint32_t p = base + tid;
@@ -759,7 +1002,7 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
     uint64_t lo = 0;
     if (p < nprobe) {
       if (use_rec) {
-        const uint64_t r = c.rec[(uint64_t)q * c.rec_stride + p];
+        const uint64_t r = rec[1 + p];
         present = r != kRecAbsent;
         if (present) {
           lo = r >> 24;
@@ -831,8 +1074,21 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
   TM::sync();
 
   uint32_t nP = 0, nranks = 0, nchains = 0, G = 0;
-  bool direct = false;
-  if (!(c.flags & MH_CLASSIFY_NO_DIRECT) && nhits == nprobe && nhits >= 3) {
+  bool direct = false, sorted = false, small = false;
+  if constexpr (BS <= 64) {
+    if (E <= 2 * BS && E <= 64 && nhits <= 64 && m < (1u << 26)) {  // qPos fits 26 bits
+      small = true;
+      const uint32_t r = small_path<TM>(c, ws, E, nhits, nprobe, nP);
+      if (r == kSmallDirect) {
+        direct = true;
+        nranks = 1;
+        nchains = 1;
+        G = nP;
+      }
+      sorted = r == kSmallSorted;
+    }
+  }
+  if (!small && !(c.flags & MH_CLASSIFY_NO_DIRECT) && nhits == nprobe && nhits >= 3) {
     // Direct hit.  Every probe found a bucket, and one reference r* has
     // exactly one entry in each bucket, all on one diagonal.  Then r* has the
     // longest run N = nhits, the cut is N - 1, and if no other reference
@@ -906,7 +1162,7 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
     }
   }
 
-  if (!direct) {
+  if (!direct && !sorted) {
     // A seed needs two pairs of one reference one stride apart on both axes
     // (a chain link).  Links only join pairs of consecutive probes, so when
     // no entry of a hit has its successor in the next probe's bucket, every
@@ -933,18 +1189,21 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
     // (counted only from 2, :2067-2072), cut = max(1, M - 1) (:2099); every
     // pair of a reference whose run reaches the cut is kept.  Result: kept
     // pairs in k2/v2 as (ref << 32 | refPos, qPos), unsorted.
-    if (nhits < (1u << 12) && ix.n_refs < (1u << 20))
-      filter_hashed<TM>(ws, E, nP);
-    else
-      filter_sorted<TM>(ws, E, nP);
-    // Sort by (reference, refPos, qPos) (Analyser.cpp:1024-1059).
-    const uint32_t p2 = next_pow2(nP);
-    TM::sync();
-    for (uint32_t e = nP + tid; e < p2; e += BS) {
-      ws.k2[e] = ~0ull;
-      ws.v2[e] = ~0u;
+    // (Done in registers by small_path for small working sets.)
+    if (!sorted) {
+      if (nhits < (1u << 12) && ix.n_refs < (1u << 20))
+        filter_hashed<TM>(ws, E, nP);
+      else
+        filter_sorted<TM>(ws, E, nP);
+      // Sort by (reference, refPos, qPos) (Analyser.cpp:1024-1059).
+      const uint32_t p2 = next_pow2(nP);
+      TM::sync();
+      for (uint32_t e = nP + tid; e < p2; e += BS) {
+        ws.k2[e] = ~0ull;
+        ws.v2[e] = ~0u;
+      }
+      bitonic_sort<TM>(ws.k2, ws.v2, p2);
     }
-    bitonic_sort<TM>(ws.k2, ws.v2, p2);
     // Candidate references in ascending order: rank = index among the
     // distinct references; k2 becomes rank << 32 | refPos.
     for (uint32_t t = tid; t < nP; t += BS)
@@ -1017,6 +1276,7 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
     return kDone;
   }
 
+  if (!staged) stage();
   // Score every reference whose longest chain equals G (Analyser.cpp:2429-2436).
   const bool ultrafast = (c.flags & MH_CLASSIFY_ULTRAFAST) != 0;
   const uint32_t *cfl = (const uint32_t *)ws.k1;
@@ -1159,8 +1419,11 @@ constexpr uint32_t kQueueChunk = 16;
 // while the queue holds at least four chunks per team, one per team for
 // short queues (the overflow tiers).  Queries that do not fit are passed on
 // to the next tier.
+// Lane-team kernels are held to 128 VGPRs (4 wavefronts per SIMD): their
+// throughput follows the number of strands in flight.
 template <class TM>
-__global__ __launch_bounds__(TM::kSize * TM::kPerWave) void k_classify(
+__global__ __launch_bounds__(TM::kSize * TM::kPerWave)
+__attribute__((amdgpu_waves_per_eu(TM::kPerWave > 1 ? 4 : 1, 8))) void k_classify(
     Ctx c, Caps cap, const uint32_t *qlist, const unsigned long long *qstart, uint32_t nq,
     const unsigned long long *nq_dev, unsigned long long *head) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1231,45 +1494,81 @@ __global__ __launch_bounds__(BS) void k_classify_hbm(Ctx c, Caps cap, unsigned c
   stats_flush<TM>(ws, c.out.ctr, blockIdx.x);
 }
 
-// Probe records for the strands of one length bucket: thread (i, p) hashes
-// window p of strand list[i] (the bases the tile kernel would stage) and looks
-// up its bucket.  The directory loads of many strands are then in flight at
-// once, and the tile kernel reads one record per probe.
-__global__ __launch_bounds__(256) void k_probe(IndexView ix, Reads rd, const uint32_t *binned,
-                                               const unsigned long long *start,
-                                               const unsigned long long *count, uint32_t stride,
-                                               uint32_t lanes_log2, uint64_t *rec) {
+// Probe records for the strands of the shortest length bucket.  An 8-lane
+// team per strand, 32 strands per 256-thread workgroup: the team copies its
+// read into an LDS row with dword loads, checks the bases against the
+// alphabet and writes the strand (uppercase, reverse complement for strand
+// 1) into a second row; then each lane hashes one probe window and looks up
+// its bucket.  All lookups of a workgroup are in flight together, so the
+// pass runs at the rate HBM serves random 16-byte reads.  Strands longer
+// than `maxlen` are left to the tile kernel.
+constexpr uint32_t kProbeLanes = 8;
+constexpr uint32_t kProbeTeams = 32;
+__global__ __launch_bounds__(kProbeLanes * kProbeTeams) void k_probe(
+    IndexView ix, Reads rd, const uint32_t *binned, const unsigned long long *start,
+    const unsigned long long *count, uint32_t stride, uint32_t row_words, uint32_t maxlen,
+    uint64_t *rec) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint8_t *comp = smem;  // 256-byte complement table
+  const uint32_t team = threadIdx.x / kProbeLanes, lane = threadIdx.x % kProbeLanes;
+  uint32_t *raw = (uint32_t *)(smem + 256) + team * 2 * row_words;  // the read's dwords
+  uint8_t *str = (uint8_t *)(raw + row_words);                       // the strand's bases
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) comp[i] = kComp[i];
   const uint32_t *list = binned + *start;
   const uint64_t n = *count;
-  const uint64_t total = n << lanes_log2;
-  const uint32_t lane_mask = (1u << lanes_log2) - 1;
+  const uint8_t *buf_end = rd.bytes + rd.off[rd.n];
   const int T = ix.T, s = ix.s, k = ix.k;
-  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-       g += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t p = (uint32_t)g & lane_mask;
-    if (p >= stride) continue;
-    const uint32_t q = list[g >> lanes_log2];
-    const uint32_t read = q >> 1, strand = q & 1;
-    const uint64_t r0 = rd.off[read];
-    const uint64_t m = rd.off[read + 1] - r0;
-    if (m <= (uint64_t)T) continue;
-    const uint64_t nprobe = (m - T + s - 1) / s;
-    if (p >= nprobe) continue;
-    const uint8_t *b = rd.bytes + r0;
-    const uint64_t w0 = (uint64_t)p * s;
-    uint32_t h = 2166136261u;  // Hashmap.cpp:3197-3211
-    if (strand == 0) {
-      for (int x = 0; x < k; x++) h = (h * 16777619u) ^ upper(b[w0 + x]);
-    } else {
-      for (int x = 0; x < k; x++) h = (h * 16777619u) ^ kComp[upper(b[m - 1 - (w0 + x)])];
+  __syncthreads();
+  for (uint64_t base = (uint64_t)blockIdx.x * kProbeTeams; base < n;
+       base += (uint64_t)gridDim.x * kProbeTeams) {
+    const uint64_t i = base + team;
+    if (i >= n) continue;  // team-uniform; no workgroup barrier below
+    const uint32_t q = list[i];
+    const uint64_t r0 = rd.off[q >> 1];
+    const uint64_t m = rd.off[(q >> 1) + 1] - r0;
+    if (m > maxlen) continue;
+    const uint32_t mm = (uint32_t)m, strand = q & 1;
+    // dwords at 4-byte aligned addresses covering the read
+    const uint8_t *first = rd.bytes + r0;
+    const uintptr_t a0 = (uintptr_t)first & ~(uintptr_t)3;
+    const uint32_t off = (uint32_t)((uintptr_t)first - a0);
+    const uint32_t nw = (off + mm + 3) >> 2;
+    for (uint32_t w = lane; w < nw; w += kProbeLanes) {
+      const uintptr_t at = a0 + 4u * w;
+      uint32_t word = 0;
+      if (at >= (uintptr_t)first && at + 4 <= (uintptr_t)buf_end) {
+        word = *(const uint32_t *)at;
+      } else {  // the first or last dword: only bytes of the buffer
+        for (uint32_t t = 0; t < 4; t++)
+          if (at + t >= (uintptr_t)rd.bytes && at + t < (uintptr_t)buf_end)
+            word |= (uint32_t)(*(const uint8_t *)(at + t)) << (8 * t);
+      }
+      raw[w] = word;
     }
-    h = ((h >> kHashBits) ^ h) & kHashMask;
-    uint64_t r = 0;  // the empty bucket: present, no entries
-    if (h != ix.empty_hash) {
-      const uint64_t lo = ix.dir[h], hi = ix.dir[h + 1];
-      r = hi > lo ? (lo << 24) | (uint64_t)min(hi - lo, (uint64_t)kRecCntMax) : kRecAbsent;
+    LaneTeam<kProbeLanes>::sync();
+    const uint8_t *b = (const uint8_t *)raw + off;
+    int bad = 0;
+    for (uint32_t x = lane; x < mm; x += kProbeLanes) {
+      const uint8_t ch = upper(b[strand ? mm - 1 - x : x]);
+      const uint8_t cc = comp[ch];  // zero outside the nucleotide alphabet
+      bad |= cc == 0;
+      str[x] = strand ? cc : ch;
     }
-    rec[(uint64_t)q * stride + p] = r;
+    uint64_t *out = rec + (uint64_t)q * stride;
+    bad = LaneTeam<kProbeLanes>::any(bad);
+    if (lane == 0) out[0] = bad ? kRecBad : 0ull;
+    if (bad || mm <= (uint32_t)T) continue;
+    const uint32_t nprobe = (mm - T + s - 1) / s;
+    for (uint32_t p = lane; p < nprobe; p += kProbeLanes) {
+      const uint32_t h = hash28(str + p * s, k);
+      uint64_t r = 0;  // the empty bucket: present, no entries
+      if (h != ix.empty_hash) {
+        const uint64_t lo = ix.dir[h], hi = ix.dir[h + 1];
+        r = hi > lo ? (lo << 24) | (uint64_t)min(hi - lo, (uint64_t)kRecCntMax) : kRecAbsent;
+      }
+      out[1 + p] = r;
+    }
+    LaneTeam<kProbeLanes>::sync();  // rows are reused by the team's next strand
   }
 }
 
@@ -1596,14 +1895,14 @@ std::vector<mh_tile> active_tiles(const mh_handle *h) {
   return h->tiles.empty() ? default_tiles(h->max_read_len) : h->tiles;
 }
 
-// Probe records per strand of the first length tile (0: that tile hashes
-// and looks up itself): with MH_CLASSIFY_PREPROBE, one per probe the tile
-// can hold.
+// Probe records per strand of the first length tile (0: that tile stages,
+// hashes and looks up itself): with MH_CLASSIFY_PREPROBE, a status word and
+// one record per probe the tile can hold, for tiles of reads up to 1 kb.
 uint32_t probe_stride(const std::vector<mh_tile> &tiles, int s, size_t lds_limit,
                       uint32_t flags) {
   if (!(flags & MH_CLASSIFY_PREPROBE)) return 0;
   const Caps c0 = tile_caps(tiles[0], s, lds_limit);
-  return c0.pcap <= 64 ? c0.pcap : 0u;
+  return c0.qcap <= 1024 ? c0.pcap + 1 : 0u;
 }
 
 // Scratch layout for one classify call (bytes from the handle's arena):
@@ -1682,6 +1981,7 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
   c.max_len = h->max_read_len;
   c.rec = nullptr;
   c.rec_stride = 0;
+  c.rec_maxlen = 0;
   c.out = a.out;
   c.out.q = qo;
   c.out.ctr = x.d_ctr;
@@ -1695,6 +1995,7 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
   const bool timed = (a.flags & MH_CLASSIFY_TIMING) != 0 && !x.capturing;
   x.timed = false;
   x.timed_buckets = 0;
+  x.timed_probe = false;
   if (timed) hipEventRecord(x.ev[0], s);
   hipLaunchKernelGGL(k_zero, dim3(8), dim3(256), 0, s, (unsigned long long *)x.d_ctr, zn);
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, bw, 64u);
@@ -1735,16 +2036,26 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
       // Probe records for the first tile's strands.
       Ctx c0 = c;
       if (rec_stride) {
+        if (timed) {
+          hipEventRecord(x.ev[12], s);
+          x.timed_probe = true;
+        }
         uint64_t *rec = (uint64_t *)(base + L.rec);
-        uint32_t lg = 0;
-        while ((1u << lg) < rec_stride) lg++;
-        const uint64_t threads = nq << lg;
-        const uint32_t blocks = (uint32_t)std::min<uint64_t>((threads + 255) / 256, 1u << 20);
-        hipLaunchKernelGGL(k_probe, dim3(blocks), dim3(256), 0, s, a.ix, rd,
-                           (const uint32_t *)binned, (const unsigned long long *)(bw + 16),
-                           (const unsigned long long *)bw, rec_stride, lg, rec);
+        const Caps c0cap = tile_caps(tiles[0], a.ix.s, h->lds_per_block);
+        const uint32_t maxlen = c0cap.qcap;
+        const uint32_t row_words = (maxlen + 3 + 3) / 4;  // the read, any alignment
+        const size_t lds = 256 + (size_t)kProbeTeams * 2 * row_words * 4;
+        const uint32_t blocks =
+            (uint32_t)std::min<uint64_t>((nq + kProbeTeams - 1) / kProbeTeams,
+                                         (uint64_t)h->cu_count * 64);
+        hipFuncSetAttribute((const void *)k_probe, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+        hipLaunchKernelGGL(k_probe, dim3(blocks), dim3(kProbeLanes * kProbeTeams), lds, s, a.ix,
+                           rd, (const uint32_t *)binned, (const unsigned long long *)(bw + 16),
+                           (const unsigned long long *)bw, rec_stride, row_words, maxlen, rec);
         c0.rec = rec;
         c0.rec_stride = rec_stride;
+        c0.rec_maxlen = maxlen;
       }
       // Long buckets first so that their tail overlaps with the short ones.
       if (timed) hipEventRecord(x.ev[9], s);

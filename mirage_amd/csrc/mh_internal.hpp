@@ -136,7 +136,9 @@ struct Exec {
   DevCounters *h_ctr = nullptr;  // pinned host copy
   // timing events: [0] start, [1..8] after bucket kernels, [9] before the
   // bucket kernels, [10] after the large kernel, [11] after the merge
-  hipEvent_t ev[12] = {};
+  hipEvent_t ev[13] = {};  // 0 start, 1-8 buckets, 9 buckets begin, 10 tiers end,
+                           // 11 end, 12 probe pre-pass begin
+  bool timed_probe = false;
   uint32_t tie_factor = 4;       // tie pool entries per query, grown on overflow
   // captured batched call (MH_CLASSIFY_GRAPH) and the arguments it was made for
   bool capturing = false;

@@ -47,7 +47,7 @@ class Counters(ctypes.Structure):
 class Timing(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_float), ("bucket_ms", ctypes.c_float * 8),
                 ("big_ms", ctypes.c_float), ("merge_ms", ctypes.c_float),
-                ("n_buckets", ctypes.c_uint32)]
+                ("n_buckets", ctypes.c_uint32), ("probe_ms", ctypes.c_float)]
 
 
 class Tile(ctypes.Structure):
@@ -190,7 +190,7 @@ class Handle:
         t = Timing()
         self.check(self.lib.mh_get_timing(self.h, ctypes.byref(t)), "mh_get_timing")
         return {"total_ms": t.total_ms, "bucket_ms": list(t.bucket_ms), "big_ms": t.big_ms,
-                "merge_ms": t.merge_ms, "n_buckets": t.n_buckets}
+                "merge_ms": t.merge_ms, "n_buckets": t.n_buckets, "probe_ms": t.probe_ms}
 
     def autotune(self, index, d_reads, d_read_off, n_reads, iters=3):
         """Times tile candidates per length bucket on device-resident reads and
