@@ -170,6 +170,19 @@ def cpu_baseline(rb, ro, qb, qo, threshold, sample, threads, gpu_out, gpu_top):
             "bit_exact": bool(same)}
 
 
+def profiled_traffic(fname, kernel, scale):
+    """(HBM bytes per launch, source) from a committed profiler summary of
+    `kernel` on the full-size workload, or None."""
+    path = os.path.join(ROOT, "profiles", fname)
+    if scale != 1.0 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        tj = json.load(f)
+    if tj.get("kernel") != kernel:
+        return None
+    return int(tj["traffic_bytes"]), tj.get("source")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -250,6 +263,13 @@ def main():
                "tile_kernel_ms": round(float(np.mean([t_["bucket_ms"][0] for t_ in ptim_])), 4),
                "probe_alg_bytes": palg,
                "probe_achieved_GBs": round(palg / (probe_ms / 1e3) / 1e9, 1)}
+        # measured HBM bytes of the probe pass (profiler counters, committed
+        # summary) over the pass time measured here
+        ptraf = profiled_traffic("k_probe_workload2_traffic.json", "k_probe", args.scale)
+        if ptraf:
+            pre["probe_traffic_bytes"] = ptraf[0]
+            pre["probe_traffic_GBs"] = round(ptraf[0] / (probe_ms / 1e3) / 1e9, 1)
+            pre["probe_traffic_frac_of_peak"] = round(pre["probe_traffic_GBs"] / HBM_PEAK_GBS, 4)
         log(f"pre-probe form: {psec_ * 1e3:.3f} ms/step, probe pass {probe_ms:.3f} ms "
             f"({pre['probe_achieved_GBs']} GB/s algorithmic), same results {same}")
         if psec_ < sec:
@@ -306,13 +326,8 @@ def main():
     # HBM bytes per launch of this kernel from the profiler's FETCH_SIZE and
     # WRITE_SIZE counters, scaled by what they read for a 1 GiB copy (the
     # summary in profiles/ says how it was measured).
-    traffic, traffic_src = None, None
-    tpath = os.path.join(ROOT, "profiles", "k_classify_lane16_workload2_traffic.json")
-    if os.path.exists(tpath) and args.scale == 1.0:
-        with open(tpath) as f:
-            tj = json.load(f)
-        if tj.get("kernel") == kernel_name:
-            traffic, traffic_src = int(tj["traffic_bytes"]), tj.get("source")
+    traffic, traffic_src = profiled_traffic("k_classify_lane16_workload2_traffic.json",
+                                            kernel_name, args.scale) or (None, None)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "traffic_source": traffic_src, "kernel": kernel_name,

@@ -134,3 +134,33 @@ def test_read_statuses():
                 np.float32(oout["highest"][j]).view(np.uint32)
             assert ties[i] == oties[j]
     h.close()
+
+
+@pytest.mark.parametrize("shift", [1, 2, 3])
+def test_preprobe_reads_at_unaligned_addresses(handle, shift):
+    """The probe pass loads reads as aligned dwords: a read buffer that starts
+    `shift` bytes past an aligned address and ends inside a dword gives the
+    same results as the oracle."""
+    import torch
+    w = D.workload(1, 0.05)
+    rb, ro, qb, qo, _ = D.generate(w)
+    if len(qb) % 4 == 0:  # make the buffer end inside a dword
+        qb, qo = qb[:-1].copy(), qo.copy()
+        qo[-1] -= 1
+    n = len(qo) - 1
+    ix = handle.build_index(rb, ro, w.threshold)
+    host = torch.zeros(len(qb) + 8, dtype=torch.uint8)
+    host[shift:shift + len(qb)] = torch.from_numpy(qb)
+    d_all = host.cuda()
+    d_reads = d_all[shift:shift + len(qb)]  # a view: data pointer + shift
+    assert d_reads.data_ptr() % 4 == shift
+    d_off = torch.from_numpy(qo.view(np.int64)).cuda()
+    d_out = torch.empty(n * native.READ_RESULT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    top_cap = 64 * n + 1024
+    d_top = torch.empty(top_cap * 4, dtype=torch.uint8, device="cuda")
+    ix.classify_device(d_reads, d_off, n, d_out, d_top, top_cap,
+                       flags=native.MH_CLASSIFY_BATCHED | native.MH_CLASSIFY_PREPROBE)
+    out = d_out.cpu().numpy().view(native.READ_RESULT_DTYPE)
+    top = d_top.cpu().numpy().view(np.uint32)
+    oout, otop, _ = O.Index(rb, ro, w.threshold).classify(qb, qo, top_cap=512)
+    compare_reads(out, top, oout, otop, f"unaligned by {shift}")
