@@ -131,18 +131,18 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t *red) {
   }
 }
 
-// A team processes one query.  BlockTeam<BS>: the whole workgroup, with its
-// wavefronts taking seeds in turn.  LaneTeam<L>: L lanes of a one-wavefront
-// workgroup, 64 / L queries side by side.  Its members exchange data through
+// A team processes one query.  BlockTeam<BS>: the whole workgroup.
+// LaneTeam<L>: L lanes of a one-wavefront workgroup, 64 / L queries side by
+// side.  Its members exchange data through
 // cross-lane operations and LDS without workgroup barriers: the LDS
 // operations of a wavefront execute in order, and the fences keep the
 // compiler from moving memory accesses across a team synchronisation.
 template <int BS>
 struct BlockTeam {
   static constexpr uint32_t kSize = BS;
-  static constexpr uint32_t kWorkers = BS / 64;  // wavefronts that take seeds
   static constexpr uint32_t kWorkerLanes = 64;
   static constexpr uint32_t kPerWave = 1;        // teams per workgroup
+  static constexpr uint32_t kSeedLanes = 16;     // lanes that extend one seed
   __device__ static uint32_t rank() { return threadIdx.x; }
   __device__ static uint32_t index() { return 0; }
   __device__ static uint32_t worker() { return threadIdx.x >> 6; }
@@ -161,10 +161,10 @@ template <int L>
 struct LaneTeam {
   static_assert(L == 8 || L == 16 || L == 32, "lane team size");
   static constexpr uint32_t kSize = L;
-  static constexpr uint32_t kWorkers = 1;
   static constexpr uint32_t kWorkerLanes = L;
   static constexpr uint32_t kPerWave = 64 / L;
   static constexpr uint64_t kMask = (1ull << L) - 1;
+  static constexpr uint32_t kSeedLanes = L < 16 ? L : 16;
   __device__ static uint32_t rank() { return threadIdx.x & (L - 1); }
   __device__ static uint32_t index() { return threadIdx.x / L; }
   __device__ static uint32_t worker() { return 0; }
@@ -202,6 +202,19 @@ struct LaneTeam {
     return v;
   }
 };
+
+// Seed workers: groups of TM::kSeedLanes lanes of a team, each extending its
+// own seeds (four per wavefront), so the memory round trips of several seeds
+// overlap.
+template <class TM>
+__device__ __forceinline__ uint32_t seed_lane() {
+  return threadIdx.x & (TM::kSeedLanes - 1);
+}
+template <class TM>
+__device__ __forceinline__ uint64_t seed_ballot(bool p) {
+  constexpr uint32_t SL = TM::kSeedLanes;
+  return (__ballot(p) >> (threadIdx.x & (64 - SL))) & ((1ull << SL) - 1);
+}
 
 __device__ __forceinline__ bool pair_less(uint64_t ka, uint32_t va, uint64_t kb, uint32_t vb) {
   return ka < kb || (ka == kb && va < vb);
@@ -377,15 +390,15 @@ __device__ __forceinline__ float matrix_score(uint64_t sum_r, uint64_t sum_c, ui
   return (float)(((double)sf / 100.0) * (double)pct);
 }
 
-// Head/tail extension of one seed by one wavefront (MatrixCPU.cpp:93-172).
+// Head/tail extension of one seed by one seed worker (MatrixCPU.cpp:93-172).
 // Returns the matched length tot and the head length h; all lanes agree.
 template <class TM>
 __device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const uint8_t *rb,
                                             uint64_t ref_len, uint32_t sref, uint32_t rs,
                                             uint32_t lref, uint32_t re, int T, bool ultrafast,
                                             uint32_t &tot, uint32_t &head, uint32_t &compares) {
-  constexpr uint32_t W = TM::kWorkerLanes;
-  const uint32_t lane = TM::wlane();
+  constexpr uint32_t W = TM::kSeedLanes;
+  const uint32_t lane = seed_lane<TM>();
   const uint32_t mid = re - rs;
   uint32_t t = 0, h = 0;
   compares = 0;
@@ -399,7 +412,7 @@ __device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const 
     for (uint32_t b = 0; b < tl; b += W) {
       const uint32_t l = b + lane;
       const bool mm = l < tl && q[re + l] != rb[(uint64_t)lref + l];
-      const unsigned long long bal = TM::wballot(mm);
+      const unsigned long long bal = seed_ballot<TM>(mm);
       if (bal) {
         t = b + (uint32_t)__ffsll((long long)bal) - 1;
         break;
@@ -417,7 +430,7 @@ __device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const 
         for (uint32_t b = 0; b <= lb; b += W) {
           const uint32_t l = b + lane;
           const bool mm = l <= lb && q[rs - 1 - l] != rb[sref - l];
-          const unsigned long long bal = TM::wballot(mm);
+          const unsigned long long bal = seed_ballot<TM>(mm);
           if (bal) {
             h = b + (uint32_t)__ffsll((long long)bal) - 1;
             break;
@@ -433,10 +446,10 @@ __device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const 
 
 template <class TM>
 __device__ __forceinline__ void set_bits(uint32_t *bm, uint32_t a, uint32_t len) {
-  const uint32_t lane = TM::wlane();
+  const uint32_t lane = seed_lane<TM>();
   const uint32_t b = a + len;  // [a, b)
   const uint32_t w0 = a >> 5, w1 = (b - 1) >> 5;
-  for (uint32_t w = w0 + lane; w <= w1; w += TM::kWorkerLanes) {
+  for (uint32_t w = w0 + lane; w <= w1; w += TM::kSeedLanes) {
     uint32_t mask = 0xFFFFFFFFu;
     if (w == w0) mask &= 0xFFFFFFFFu << (a & 31);
     if (w == w1) mask &= 0xFFFFFFFFu >> (31 - ((b - 1) & 31));
@@ -450,8 +463,8 @@ template <class TM, class GetSeed>
 __device__ void score_ref(const uint8_t *q, uint32_t m, const uint8_t *rb, uint64_t ref_len,
                           uint32_t n_seeds, int T, bool ultrafast, const Ws &ws,
                           GetSeed get_seed, uint64_t &sum_r, uint64_t &sum_c, uint32_t &hits) {
-  constexpr uint32_t NW = TM::kWorkers, BS = TM::kSize;
-  const uint32_t wid = TM::worker(), lane = TM::wlane();
+  constexpr uint32_t NW = TM::kSize / TM::kSeedLanes, BS = TM::kSize;
+  const uint32_t wid = TM::rank() / TM::kSeedLanes, lane = seed_lane<TM>();
   const uint32_t nw = (m + 31) >> 5;
   TM::sync();
   for (uint32_t w = TM::rank(); w < nw; w += BS) ws.bitmap[w] = 0;
