@@ -216,14 +216,22 @@ __device__ __forceinline__ uint64_t seed_ballot(bool p) {
   return (__ballot(p) >> (threadIdx.x & (64 - SL))) & ((1ull << SL) - 1);
 }
 
+__device__ __forceinline__ uint32_t next_pow2(uint32_t x) {
+  return x <= 1 ? 1u : 1u << (32 - __clz(x - 1));
+}
+
 __device__ __forceinline__ bool pair_less(uint64_t ka, uint32_t va, uint64_t kb, uint32_t vb) {
   return ka < kb || (ka == kb && va < vb);
 }
 
-// Ascending sort of (key, val) pairs over n (power of two) slots.  Up to one
-// wavefront's worth of slots (64, or the team's lanes) are ranked through
-// shuffles: each lane counts the smaller elements and writes its element to
-// its rank.  Larger arrays use a bitonic network in LDS.
+// Ascending sort of n (key, val) pairs, any n.  Up to one wavefront's worth
+// of elements (64, or the team's lanes) are ranked through shuffles: each
+// lane counts the smaller elements and writes its element to its rank.
+// Larger arrays use a bitonic network in LDS whose comparators all put the
+// smaller element first: each merge of size k compares i with its mirror
+// i ^ (k - 1), then with i ^ d for d = k/4 .. 1.  Positions n and beyond act
+// as +infinity and never move, so the comparisons that would touch them are
+// skipped and nothing is padded.
 template <class TM>
 __device__ void bitonic_sort(uint64_t *key, uint32_t *val, uint32_t n) {
   constexpr uint32_t W = TM::kWorkerLanes;
@@ -249,18 +257,19 @@ __device__ void bitonic_sort(uint64_t *key, uint32_t *val, uint32_t n) {
     TM::sync();
     return;
   }
-  for (uint32_t size = 2; size <= n; size <<= 1) {
+  const uint32_t n2 = next_pow2(n);
+  for (uint32_t size = 2; size <= n2; size <<= 1) {
     for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
       TM::sync();
       const uint32_t lg = 31 - __clz(stride);
-      for (uint32_t t = TM::rank(); t < (n >> 1); t += TM::kSize) {
+      const bool mirror = stride == (size >> 1);
+      for (uint32_t t = TM::rank(); t < (n2 >> 1); t += TM::kSize) {
         const uint32_t i = ((t >> lg) << (lg + 1)) + (t & (stride - 1));
-        const uint32_t j = i + stride;
-        const bool asc = (i & size) == 0;
+        const uint32_t j = mirror ? (i ^ (size - 1)) : (i + stride);
+        if (j >= n) continue;  // j > i: +infinity at j, nothing moves
         const uint64_t ka = key[i], kb = key[j];
         const uint32_t va = val[i], vb = val[j];
-        const bool sw = asc ? pair_less(kb, vb, ka, va) : pair_less(ka, va, kb, vb);
-        if (sw) {
+        if (pair_less(kb, vb, ka, va)) {
           key[i] = kb;
           key[j] = ka;
           val[i] = vb;
@@ -272,9 +281,6 @@ __device__ void bitonic_sort(uint64_t *key, uint32_t *val, uint32_t n) {
   TM::sync();
 }
 
-__device__ __forceinline__ uint32_t next_pow2(uint32_t x) {
-  return x <= 1 ? 1u : 1u << (32 - __clz(x - 1));
-}
 
 // Workspace carved from LDS (or HBM for the large kernel).
 struct Ws {
@@ -581,10 +587,13 @@ __device__ uint32_t run_lengths(const Ws &ws, const uint64_t *key, uint32_t n, u
 // keys in t3..rref (2E slots each); the keys are free again after the
 // inserts and hold the run scratch.
 template <class TM>
-__device__ void filter_hashed(const Ws &ws, uint32_t E, uint32_t &nP) {
+__device__ void filter_hashed(const Ws &ws, uint32_t E, uint32_t ecap, uint32_t &nP) {
   constexpr uint32_t BS = TM::kSize;
   const uint32_t tid = TM::rank();
-  const uint32_t cap = next_pow2(2 * E), mask = cap - 1;
+  // 2E slots, at most the largest power of two in the 2 * ecap words (more
+  // than E slots either way)
+  const uint32_t room = 1u << (31 - __clz(2 * ecap));
+  const uint32_t cap = min(next_pow2(2 * E), room), mask = cap - 1;
   uint32_t *rval = ws.t1, *rkeys = ws.t3;
   for (uint32_t i = tid; i < cap; i += BS) {
     rkeys[i] = kEmpty;
@@ -620,12 +629,10 @@ __device__ void filter_hashed(const Ws &ws, uint32_t E, uint32_t &nP) {
   TM::sync();
   // Every reference starts with a run of 1; the sorted pass raises it.
   for (uint32_t i = tid; i < cap; i += BS) rval[i] = 1;
-  const uint32_t n2 = next_pow2(nm);
-  for (uint32_t e = nm + tid; e < n2; e += BS) ws.k2[e] = ~0ull;
   TM::sync();
   uint32_t maxrun = 0;
   if (nm) {
-    bitonic_sort<TM>(ws.k2, ws.t3, n2);  // t3: value scratch, unused
+    bitonic_sort<TM>(ws.k2, ws.t3, nm);  // t3: value scratch, unused
     maxrun = run_lengths<TM, 32, 12>(ws, ws.k2, nm, ws.t3, ws.rref);
     for (uint32_t t = tid; t < nm; t += BS) atomicMax(&rval[(uint32_t)ws.k2[t]], ws.t3[t]);
     TM::sync();
@@ -663,12 +670,7 @@ template <class TM>
 __device__ void filter_sorted(const Ws &ws, uint32_t E, uint32_t &nP) {
   constexpr uint32_t BS = TM::kSize;
   const uint32_t tid = TM::rank();
-  const uint32_t n2 = next_pow2(E);
-  for (uint32_t e = E + tid; e < n2; e += BS) {
-    ws.k1[e] = ~0ull;
-    ws.v1[e] = ~0u;
-  }
-  bitonic_sort<TM>(ws.k1, ws.v1, n2);
+  bitonic_sort<TM>(ws.k1, ws.v1, E);
   const uint32_t maxrun = run_lengths<TM, 0, 32>(ws, ws.k1, E, ws.t2, ws.t3);
   const uint32_t cut = maxrun >= 2 ? maxrun - 1 : 1;
   // longest run per reference segment
@@ -854,7 +856,6 @@ __device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t 
   }
   // Segment heads (a new reference), then each reference's hit mask: a
   // segmented OR over its elements, forward and backward.
-  const uint32_t pk_lo = __shfl_up((uint32_t)key[1], 1, W);
   const uint32_t pk_hi = __shfl_up((uint32_t)(key[1] >> 32), 1, W);
   bool head[2], valid[2];
   uint64_t bit[2];
@@ -1205,17 +1206,12 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
     // (Done in registers by small_path for small working sets.)
     if (!sorted) {
       if (nhits < (1u << 12) && ix.n_refs < (1u << 20))
-        filter_hashed<TM>(ws, E, nP);
+        filter_hashed<TM>(ws, E, cap.ecap, nP);
       else
         filter_sorted<TM>(ws, E, nP);
       // Sort by (reference, refPos, qPos) (Analyser.cpp:1024-1059).
-      const uint32_t p2 = next_pow2(nP);
       TM::sync();
-      for (uint32_t e = nP + tid; e < p2; e += BS) {
-        ws.k2[e] = ~0ull;
-        ws.v2[e] = ~0u;
-      }
-      bitonic_sort<TM>(ws.k2, ws.v2, p2);
+      bitonic_sort<TM>(ws.k2, ws.v2, nP);
     }
     // Candidate references in ascending order: rank = index among the
     // distinct references; k2 becomes rank << 32 | refPos.
@@ -1841,15 +1837,23 @@ __global__ void k_max_len(Reads rd, unsigned long long *out) {
 }
 
 // LDS capacities of a tile, shrunk until the workspace fits `limit` bytes:
-// first the entry capacity (down to 256), then the query length.  Queries
-// beyond the capacities go to the large workspace.
+// first the entry capacity (to what fits, multiples of 16, down to 256),
+// then the query length.  Queries beyond the capacities go to the overflow
+// tiers.
 Caps caps_for(const mh_tile &t, int s, size_t limit) {
   Caps c;
   const uint32_t sd = (uint32_t)(s > 0 ? s : 1);
   c.qcap = (t.max_len + 63) & ~63u;
   c.pcap = c.qcap / sd + 2;
   c.ecap = t.entries_cap;
-  while (ws_bytes(c) > limit && c.ecap > 256) c.ecap >>= 1;
+  if (ws_bytes(c) > limit) {
+    Caps z = c;
+    z.ecap = 0;
+    const size_t fixed = ws_bytes(z);
+    const size_t fit = fixed < limit ? (limit - fixed) / 40 : 0;  // 40 bytes per entry
+    c.ecap = (uint32_t)std::max<size_t>(256, std::min<size_t>(c.ecap, fit) & ~size_t(15));
+    while (ws_bytes(c) > limit && c.ecap > 256) c.ecap -= 16;
+  }
   while (ws_bytes(c) > limit && c.qcap > 64) {
     c.qcap = ((c.qcap / 2) + 63) & ~63u;
     c.pcap = c.qcap / sd + 2;
@@ -1868,7 +1872,10 @@ constexpr uint32_t kBigEcap = 1u << 19;
 // Capacities of a tile's team workspace: the workgroup's LDS is shared by
 // its teams.
 Caps tile_caps(const mh_tile &t, int s, size_t lds_per_block) {
-  return caps_for(t, s, lds_per_block / teams_of(t.block));
+  // 1 KB stays free for the kernel's static LDS and the 16-byte rounding of
+  // each team's workspace
+  const size_t usable = lds_per_block > 2048 ? lds_per_block - 1024 : lds_per_block / 2;
+  return caps_for(t, s, usable / teams_of(t.block));
 }
 
 // LDS of one workgroup of a tile.
