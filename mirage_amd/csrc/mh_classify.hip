@@ -1897,8 +1897,9 @@ std::vector<mh_tile> default_tiles(uint32_t max_read_len) {
   // Entry capacities follow the expected bucket entries of each length
   // (about two per probe on a 5000-reference index at k = 16); short reads
   // run 2 or 4 to a wavefront.
-  const mh_tile base[6] = {{200, 16, 32, 0},   {400, 32, 64, 0},    {1024, 128, 512, 0},
-                           {2048, 128, 512, 0}, {4096, 128, 1024, 0}, {0xFFFFFFFFu, 1024, 2048, 0}};
+  const mh_tile base[7] = {{200, 16, 32, 0},    {400, 32, 64, 0},    {1024, 128, 512, 0},
+                           {2048, 128, 512, 0},  {4096, 256, 1024, 0}, {8192, 512, 1024, 0},
+                           {0xFFFFFFFFu, 1024, 2048, 0}};
   std::vector<mh_tile> t;
   for (mh_tile b : base) {
     if (b.max_len >= max_read_len) {
