@@ -49,6 +49,35 @@ __device__ __forceinline__ uint32_t hash28(const uint8_t *p, int k) {
   return ((h >> kHashBits) ^ h) & kHashMask;
 }
 
+// Cross-lane reads through ds_bpermute with the source address computed once
+// for all the values a step moves (each HIP shuffle recomputes it).  A lane
+// whose source lane is disabled reads 0, so permutes are issued where every
+// lane of the team is active, and results are selected afterwards.
+__device__ __forceinline__ uint32_t lane64() { return __lane_id(); }
+__device__ __forceinline__ uint32_t src_xor(uint32_t mask) { return (lane64() ^ mask) << 2; }
+// source `d` lanes below / above within segments of W lanes (own lane where
+// there is none)
+template <uint32_t W>
+__device__ __forceinline__ uint32_t src_up(uint32_t d) {
+  const uint32_t l = lane64();
+  return ((l & (W - 1)) >= d ? l - d : l) << 2;
+}
+template <uint32_t W>
+__device__ __forceinline__ uint32_t src_down(uint32_t d) {
+  const uint32_t l = lane64();
+  return ((l & (W - 1)) + d < W ? l + d : l) << 2;
+}
+template <uint32_t W>
+__device__ __forceinline__ uint32_t src_at(uint32_t u) {  // lane u of the segment
+  return ((lane64() & ~(W - 1)) + u) << 2;
+}
+__device__ __forceinline__ uint32_t bperm(uint32_t src, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_ds_bpermute((int)src, (int)v);
+}
+__device__ __forceinline__ uint64_t bperm64(uint32_t src, uint64_t v) {
+  return ((uint64_t)bperm(src, (uint32_t)(v >> 32)) << 32) | bperm(src, (uint32_t)v);
+}
+
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -184,21 +213,21 @@ struct LaneTeam {
   __device__ static uint32_t excl_scan(uint32_t v, uint32_t *, uint32_t &tot) {
     uint32_t inc = v;
 #pragma unroll
-    for (int d = 1; d < L; d <<= 1) {
-      const uint32_t t = __shfl_up(inc, d, L);
-      if (rank() >= (uint32_t)d) inc += t;
+    for (uint32_t d = 1; d < L; d <<= 1) {
+      const uint32_t t = bperm(src_up<L>(d), inc);
+      inc += rank() >= d ? t : 0u;
     }
-    tot = __shfl(inc, L - 1, L);
+    tot = bperm(src_at<L>(L - 1), inc);
     return inc - v;
   }
   __device__ static uint32_t max(uint32_t v, uint32_t *) {
 #pragma unroll
-    for (int d = L / 2; d > 0; d >>= 1) v = ::max(v, (uint32_t)__shfl_xor(v, d, L));
+    for (uint32_t d = L / 2; d > 0; d >>= 1) v = ::max(v, bperm(src_xor(d), v));
     return v;
   }
   __device__ static uint32_t sum(uint32_t v, uint32_t *) {
 #pragma unroll
-    for (int d = L / 2; d > 0; d >>= 1) v += (uint32_t)__shfl_xor(v, d, L);
+    for (uint32_t d = L / 2; d > 0; d >>= 1) v += bperm(src_xor(d), v);
     return v;
   }
 };
@@ -243,9 +272,9 @@ __device__ void bitonic_sort(uint64_t *key, uint32_t *val, uint32_t n) {
       uint32_t v = lane < n ? val[lane] : ~0u;
       uint32_t rank = 0;
       for (uint32_t u = 0; u < n; u++) {
-        const uint32_t klo = __shfl((uint32_t)k, u, W), khi = __shfl((uint32_t)(k >> 32), u, W);
-        const uint32_t vu = __shfl(v, u, W);
-        const uint64_t ku = ((uint64_t)khi << 32) | klo;
+        const uint32_t src = src_at<W>(u);
+        const uint64_t ku = bperm64(src, k);
+        const uint32_t vu = bperm(src, v);
         rank += (pair_less(ku, vu, k, v) || (ku == k && vu == v && u < lane)) ? 1u : 0u;
       }
       __builtin_amdgcn_wave_barrier();
@@ -720,10 +749,7 @@ template <class TM>
 __device__ __forceinline__ uint64_t team_or64(uint64_t x) {
   constexpr int W = TM::kWorkerLanes;
 #pragma unroll
-  for (int d = W / 2; d > 0; d >>= 1) {
-    const uint32_t lo = __shfl_xor((uint32_t)x, d, W), hi = __shfl_xor((uint32_t)(x >> 32), d, W);
-    x |= ((uint64_t)hi << 32) | lo;
-  }
+  for (uint32_t d = W / 2; d > 0; d >>= 1) x |= bperm64(src_xor(d), x);
   return x;
 }
 
@@ -829,12 +855,11 @@ __device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t 
       const uint32_t lx = d >> 1;  // partner lane = lane ^ lx
       const bool lower = (lane & lx) == 0;
       const bool asc = ((2 * lane) & k) == 0;
+      const uint32_t src = src_xor(lx);
 #pragma unroll
       for (int j = 0; j < 2; j++) {
-        const uint32_t plo = __shfl_xor((uint32_t)key[j], lx, W);
-        const uint32_t phi = __shfl_xor((uint32_t)(key[j] >> 32), lx, W);
-        const uint32_t pv = __shfl_xor(v[j], lx, W);
-        const uint64_t pk = ((uint64_t)phi << 32) | plo;
+        const uint64_t pk = bperm64(src, key[j]);
+        const uint32_t pv = bperm(src, v[j]);
         const bool pgt = elem_gt(pk, pv, key[j], v[j]);
         const bool take = (lower == asc) ? !pgt && (pk != key[j] || pv != v[j]) : pgt;
         if (take) {
@@ -856,7 +881,8 @@ __device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t 
   }
   // Segment heads (a new reference), then each reference's hit mask: a
   // segmented OR over its elements, forward and backward.
-  const uint32_t pk_hi = __shfl_up((uint32_t)(key[1] >> 32), 1, W);
+  const uint32_t up1 = src_up<W>(1), down1 = src_down<W>(1);
+  const uint32_t pk_hi = bperm(up1, (uint32_t)(key[1] >> 32));
   bool head[2], valid[2];
   uint64_t bit[2];
 #pragma unroll
@@ -872,34 +898,35 @@ __device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t 
   uint64_t fv = f1;
 #pragma unroll
   for (uint32_t d = 1; d < W; d <<= 1) {
-    const uint32_t lo = __shfl_up((uint32_t)fv, d, W), hi = __shfl_up((uint32_t)(fv >> 32), d, W);
-    const int pf = __shfl_up((int)ff, d, W);
-    if (lane >= d) {
-      if (!ff) fv |= ((uint64_t)hi << 32) | lo;
-      ff = ff || pf;
-    }
+    const uint32_t src = src_up<W>(d);
+    const uint64_t pv = bperm64(src, fv);
+    const uint32_t pf = bperm(src, ff ? 1u : 0u);
+    const bool in = lane >= d;
+    fv = (in && !ff) ? (fv | pv) : fv;
+    ff = ff || (in && pf);
   }
-  const uint32_t xlo = __shfl_up((uint32_t)fv, 1, W), xhi = __shfl_up((uint32_t)(fv >> 32), 1, W);
-  const uint64_t before = lane == 0 ? 0ull : (((uint64_t)xhi << 32) | xlo);
+  // every lane takes part in a permute (a disabled source lane reads as 0)
+  const uint64_t fv_up = bperm64(up1, fv);
+  const uint64_t before = lane == 0 ? 0ull : fv_up;
   const uint64_t inc0 = head[0] ? bit[0] : (before | bit[0]);
   const uint64_t inc1 = head[1] ? bit[1] : (inc0 | bit[1]);
   // backward inclusive (a segment ends where the next element is a head)
-  const int nh0 = __shfl_down((int)head[0], 1, W);
+  const uint32_t nh0 = bperm(down1, head[0] ? 1u : 0u);
   const bool tail1 = lane == W - 1 || nh0;
   const bool tail0 = head[1];
   uint64_t bv = tail0 ? bit[0] : (bit[0] | bit[1]);
   bool bf = tail0 || tail1;
 #pragma unroll
   for (uint32_t d = 1; d < W; d <<= 1) {
-    const uint32_t lo = __shfl_down((uint32_t)bv, d, W), hi = __shfl_down((uint32_t)(bv >> 32), d, W);
-    const int pf = __shfl_down((int)bf, d, W);
-    if (lane + d < W) {
-      if (!bf) bv |= ((uint64_t)hi << 32) | lo;
-      bf = bf || pf;
-    }
+    const uint32_t src = src_down<W>(d);
+    const uint64_t pv = bperm64(src, bv);
+    const uint32_t pf = bperm(src, bf ? 1u : 0u);
+    const bool in = lane + d < W;
+    bv = (in && !bf) ? (bv | pv) : bv;
+    bf = bf || (in && pf);
   }
-  const uint32_t ylo = __shfl_down((uint32_t)bv, 1, W), yhi = __shfl_down((uint32_t)(bv >> 32), 1, W);
-  const uint64_t after = lane == W - 1 ? 0ull : (((uint64_t)yhi << 32) | ylo);
+  const uint64_t bv_down = bperm64(down1, bv);
+  const uint64_t after = lane == W - 1 ? 0ull : bv_down;
   const uint64_t suf1 = tail1 ? bit[1] : (after | bit[1]);
   const uint64_t suf0 = tail0 ? bit[0] : (suf1 | bit[0]);
   const uint64_t mask[2] = {inc0 | suf0, inc1 | suf1};
