@@ -966,7 +966,7 @@ __device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t 
 // exceeds the workspace capacities; nothing is written in that case.
 template <class TM>
 __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, uint32_t q,
-                                  bool last_resort) {
+                                  bool last_resort, const uint8_t *comp) {
   constexpr uint32_t BS = TM::kSize;
   const uint32_t tid = TM::rank();
   const uint32_t read = q >> 1, strand = q & 1;
@@ -998,13 +998,36 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
   // Stage the strand: uppercase, reverse complement for strand 1
   // (Sequences.cpp:43-52, :795-841).  With probe records the base check is
   // done and staging waits until a seed needs the bases.
+  // The read is loaded as 4-byte aligned dwords (bytes one at a time at the
+  // buffer's ends); comp is the complement table in LDS, zero outside the
+  // nucleotide alphabet.
   auto stage = [&]() {
+    const uint8_t *first = c.rd.bytes + r0;
+    const uintptr_t a0 = (uintptr_t)first & ~(uintptr_t)3;
+    const uint32_t off = (uint32_t)((uintptr_t)first - a0);
+    const uint32_t nw = (off + m + 3) >> 2;
+    const uintptr_t lo = (uintptr_t)c.rd.bytes, hi = lo + c.rd.off[c.rd.n];
     int b = 0;
-    for (uint32_t x = tid; x < m; x += BS) {
-      uint8_t ch = upper(c.rd.bytes[r0 + (strand ? (m - 1 - x) : x)]);
-      const uint8_t cc = kComp[ch];  // zero outside the nucleotide alphabet
-      b |= (cc == 0);
-      ws.q[x] = strand ? cc : ch;
+    for (uint32_t w = tid; w < nw; w += BS) {
+      const uintptr_t at = a0 + 4u * w;
+      uint32_t word = 0;
+      if (at >= lo && at + 4 <= hi) {
+        word = *(const uint32_t *)at;
+      } else {
+        for (uint32_t t = 0; t < 4; t++)
+          if (at + t >= lo && at + t < hi) word |= (uint32_t)(*(const uint8_t *)(at + t)) << (8 * t);
+      }
+#pragma unroll
+      for (uint32_t t = 0; t < 4; t++) {
+        const uint32_t x = 4 * w + t - off;  // position in the read (wraps below 0)
+        if (x < m) {
+          const uint8_t ch = upper((uint8_t)(word >> (8 * t)));
+          const uint8_t cc = comp[ch];
+          b |= (cc == 0);
+          if (strand) ws.q[m - 1 - x] = cc;
+          else ws.q[x] = ch;
+        }
+      }
     }
     return TM::any(b);
   };
@@ -1467,13 +1490,16 @@ __attribute__((amdgpu_waves_per_eu(TM::kPerWave > 1 ? 4 : 1, 8))) void k_classif
   const uint32_t team = TM::index();
   const Ws ws = ws_carve(smem + team * ws_stride_lds(cap), cap);
   __shared__ uint32_t s_base;
+  __shared__ uint8_t s_comp[256];
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_comp[i] = kComp[i];
+  __syncthreads();
   if (qlist && qstart) qlist += *qstart;
   stats_begin<TM>(ws);
   const uint32_t slot = blockIdx.x * G + team;
   if (head == nullptr) {
     if (slot < nq) {
       const uint32_t q = qlist ? qlist[slot] : slot;
-      if (process_query<TM>(c, ws, cap, q, false) == kOverflow) pass_on<TM>(c, q);
+      if (process_query<TM>(c, ws, cap, q, false, s_comp) == kOverflow) pass_on<TM>(c, q);
     }
     stats_flush<TM>(ws, c.out.ctr, slot);
     return;
@@ -1496,7 +1522,7 @@ __attribute__((amdgpu_waves_per_eu(TM::kPerWave > 1 ? 4 : 1, 8))) void k_classif
     const uint32_t end = (uint32_t)min((unsigned long long)base + chunk, n);
     for (uint32_t i = base + team; i < end; i += G) {
       const uint32_t q = qlist ? qlist[i] : i;
-      if (process_query<TM>(c, ws, cap, q, false) == kOverflow) pass_on<TM>(c, q);
+      if (process_query<TM>(c, ws, cap, q, false, s_comp) == kOverflow) pass_on<TM>(c, q);
     }
   }
   stats_flush<TM>(ws, c.out.ctr, slot);
@@ -1514,6 +1540,9 @@ __global__ __launch_bounds__(BS) void k_classify_hbm(Ctx c, Caps cap, unsigned c
   using TM = BlockTeam<BS>;
   const Ws ws = ws_carve(scratch + blockIdx.x * ws_stride, cap);
   __shared__ uint32_t slot;
+  __shared__ uint8_t s_comp[256];
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_comp[i] = kComp[i];
+  __syncthreads();
   stats_begin<TM>(ws);
   for (;;) {
     __syncthreads();
@@ -1525,7 +1554,8 @@ __global__ __launch_bounds__(BS) void k_classify_hbm(Ctx c, Caps cap, unsigned c
     __syncthreads();
     if (slot == 0xFFFFFFFFu) break;
     const uint32_t q = list[slot];
-    if (process_query<TM>(c, ws, cap, q, last_resort != 0) == kOverflow) pass_on<TM>(c, q);
+    if (process_query<TM>(c, ws, cap, q, last_resort != 0, s_comp) == kOverflow)
+      pass_on<TM>(c, q);
   }
   stats_flush<TM>(ws, c.out.ctr, blockIdx.x);
 }
