@@ -1706,14 +1706,35 @@ __device__ __forceinline__ uint32_t bucket_of(const Reads &rd, uint32_t r, const
   return b;
 }
 
+// Binning: each workgroup takes a contiguous chunk of reads; a wavefront
+// counts its reads per bucket with ballots, one LDS atomic per bucket and
+// wavefront, and the workgroup makes one global atomic per bucket.
+constexpr uint32_t kBinBlocks = 512;
+
+__device__ __forceinline__ void bin_chunk(const Reads &rd, uint32_t &lo, uint32_t &hi) {
+  const uint32_t per = (rd.n + gridDim.x - 1) / gridDim.x;
+  lo = min(rd.n, blockIdx.x * per);
+  hi = min(rd.n, lo + per);
+}
+
 __global__ void k_bin_count(Reads rd, Lens max_len, uint32_t nb, unsigned long long *cnt) {
   __shared__ unsigned int loc[8];
   if (threadIdx.x < 8) loc[threadIdx.x] = 0;
   __syncthreads();
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < rd.n) atomicAdd(&loc[bucket_of(rd, r, max_len, nb)], 2u);
+  uint32_t lo, hi;
+  bin_chunk(rd, lo, hi);
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t r0 = lo; r0 < hi; r0 += blockDim.x) {
+    const uint32_t r = r0 + threadIdx.x;
+    const uint32_t b = r < hi ? bucket_of(rd, r, max_len, nb) : 0xFFu;
+    for (uint32_t k = 0; k < nb; k++) {
+      const unsigned long long m = __ballot(b == k);
+      if (m && lane == (uint32_t)__ffsll((long long)m) - 1) atomicAdd(&loc[k], 2u * __popcll(m));
+    }
+  }
   __syncthreads();
-  if (threadIdx.x < nb && loc[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], (unsigned long long)loc[threadIdx.x]);
+  if (threadIdx.x < nb && loc[threadIdx.x])
+    atomicAdd(&cnt[threadIdx.x], (unsigned long long)loc[threadIdx.x]);
 }
 
 __global__ void k_bin_scatter(Reads rd, Lens max_len, uint32_t nb,
@@ -1723,20 +1744,43 @@ __global__ void k_bin_scatter(Reads rd, Lens max_len, uint32_t nb,
   __shared__ unsigned long long base[8];
   if (threadIdx.x < 8) loc[threadIdx.x] = 0;
   __syncthreads();
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t b = 0, idx = 0;
-  if (r < rd.n) {
-    b = bucket_of(rd, r, max_len, nb);
-    idx = atomicAdd(&loc[b], 2u);
+  uint32_t lo, hi;
+  bin_chunk(rd, lo, hi);
+  const uint32_t lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1;
+  // pass 1: the chunk's count per bucket
+  for (uint32_t r0 = lo; r0 < hi; r0 += blockDim.x) {
+    const uint32_t r = r0 + threadIdx.x;
+    const uint32_t b = r < hi ? bucket_of(rd, r, max_len, nb) : 0xFFu;
+    for (uint32_t k = 0; k < nb; k++) {
+      const unsigned long long m = __ballot(b == k);
+      if (m && lane == (uint32_t)__ffsll((long long)m) - 1) atomicAdd(&loc[k], 2u * __popcll(m));
+    }
   }
   __syncthreads();
-  if (threadIdx.x < nb && loc[threadIdx.x])
-    base[threadIdx.x] = start[threadIdx.x] +
-                        atomicAdd(&cursor[threadIdx.x], (unsigned long long)loc[threadIdx.x]);
+  if (threadIdx.x < nb)
+    base[threadIdx.x] = loc[threadIdx.x] ? start[threadIdx.x] + atomicAdd(&cursor[threadIdx.x],
+                                                                         (unsigned long long)loc[threadIdx.x])
+                                         : 0ull;
   __syncthreads();
-  if (r < rd.n) {
-    list[base[b] + idx] = 2 * r;
-    list[base[b] + idx + 1] = 2 * r + 1;
+  if (threadIdx.x < 8) loc[threadIdx.x] = 0;
+  __syncthreads();
+  // pass 2: places within the chunk's ranges
+  for (uint32_t r0 = lo; r0 < hi; r0 += blockDim.x) {
+    const uint32_t r = r0 + threadIdx.x;
+    const uint32_t b = r < hi ? bucket_of(rd, r, max_len, nb) : 0xFFu;
+    uint64_t at = 0;
+    for (uint32_t k = 0; k < nb; k++) {
+      const unsigned long long m = __ballot(b == k);
+      uint32_t first = 0;
+      if (m && lane == (uint32_t)__ffsll((long long)m) - 1) first = atomicAdd(&loc[k], 2u * __popcll(m));
+      first = __shfl(first, __ffsll((long long)m) - 1 < 0 ? 0 : __ffsll((long long)m) - 1, 64);
+      if (b == k) at = base[k] + first + 2u * __popcll(m & below);
+    }
+    if (r < hi) {
+      list[at] = 2 * r;
+      list[at + 1] = 2 * r + 1;
+    }
   }
 }
 
@@ -2107,9 +2151,10 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
       Lens d_maxlen{};
       for (uint32_t i = 0; i < nb; i++) d_maxlen.v[i] = tiles[i].max_len;
       const Reads rd = a.rd;
-      hipLaunchKernelGGL(k_bin_count, dim3(gr), dim3(256), 0, s, rd, d_maxlen, nb, bw);
+      const uint32_t bin_grid = std::min<uint32_t>(gr, kBinBlocks);
+      hipLaunchKernelGGL(k_bin_count, dim3(bin_grid), dim3(256), 0, s, rd, d_maxlen, nb, bw);
       hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(64), 0, s, bw, nb, bw + 16);
-      hipLaunchKernelGGL(k_bin_scatter, dim3(gr), dim3(256), 0, s, rd, d_maxlen, nb, bw + 16,
+      hipLaunchKernelGGL(k_bin_scatter, dim3(bin_grid), dim3(256), 0, s, rd, d_maxlen, nb, bw + 16,
                          bw + 8, binned);
       // Probe records for the first tile's strands.
       Ctx c0 = c;
