@@ -144,30 +144,48 @@ def serving_leg(native, D, scale, n_requests, loads, log):
     return out
 
 
-def cpu_baseline(rb, ro, qb, qo, threshold, sample, threads, gpu_out, gpu_top):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(rb, ro, qb, qo, threshold, sample, threads, gpu_out, gpu_top, sample1):
     """The oracle (C++ restatement of the reference CPU path) on a bounded
-    sample of the same reads; also checks the GPU results on that sample."""
+    sample of the same reads: one warm-up, then the median of 5 runs with
+    `threads` threads, and the same on a smaller sample with 1 thread.  Also
+    checks the GPU results on the sample."""
     from oracle import oracle as O
     from mirage_amd import native
     t = time.perf_counter()
     oix = O.Index(rb, ro, threshold)
     build_s = time.perf_counter() - t
+
+    def timed(n, nt):
+        sq, so = qb[: int(qo[n])], qo[: n + 1]
+        oix.classify(sq, so, n_threads=nt)  # warm-up
+        runs, res = [], None
+        for _ in range(5):
+            t0 = time.perf_counter()
+            res = oix.classify(sq, so, n_threads=nt)
+            runs.append(time.perf_counter() - t0)
+        return float(np.median(runs)), res
+
     n = min(sample, len(qo) - 1)
-    sq = qb[: int(qo[n])]
-    so = qo[: n + 1]
-    oix.classify(sq, so, n_threads=threads)  # warm-up
-    best = None
-    for _ in range(3):
-        t = time.perf_counter()
-        out, top, _ = oix.classify(sq, so, n_threads=threads)
-        dt = time.perf_counter() - t
-        best = dt if best is None else min(best, dt)
+    med, (out, top, _) = timed(n, threads)
+    n1 = min(sample1, len(qo) - 1)
+    med1, _ = timed(n1, 1)
     g = gpu_out[:n]
     same = (np.array_equal(g["highest"].view(np.uint32), out["highest"].view(np.uint32))
             and np.array_equal(g["strand"], out["strand"])
             and native.tie_sets(g, gpu_top) == O.top_sets(out, top))
-    return {"reads_per_s": n / best, "seconds": best, "reads": n, "index_build_s": build_s,
-            "bit_exact": bool(same)}
+    return {"reads_per_s": n / med, "seconds": med, "reads": n, "index_build_s": build_s,
+            "bit_exact": bool(same), "reads_per_s_1thread": n1 / med1, "reads_1thread": n1}
 
 
 def profiled_traffic(fname, kernel, scale):
@@ -342,12 +360,17 @@ def main():
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: oracle on {min(args.cpu_sample, n)} reads, {threads} threads")
-        cb = cpu_baseline(rb, ro, qb, qo, w.threshold, args.cpu_sample, threads, out, top)
+        cb = cpu_baseline(rb, ro, qb, qo, w.threshold, args.cpu_sample, threads, out, top,
+                          args.cpu_sample // 10)
         cpu = {"value": round(cb["reads_per_s"], 1), "unit": "reads/s", "cores": threads,
                "kind": "port",
                "sample": f"first {cb['reads']} reads of the same workload against the full "
-                         f"index (oracle index build {cb['index_build_s']:.1f} s, not timed)",
-               "bit_exact_vs_gpu": cb["bit_exact"]}
+                         f"index, median of 5 after 1 warm-up (oracle index build "
+                         f"{cb['index_build_s']:.1f} s, not timed)",
+               "bit_exact_vs_gpu": cb["bit_exact"],
+               "value_1thread": round(cb["reads_per_s_1thread"], 1),
+               "sample_1thread": f"first {cb['reads_1thread']} reads, median of 5",
+               "host_nproc": os.cpu_count(), "host_cpu": cpu_model()}
         log(f"cpu: {cb['reads_per_s'] / 1e3:.1f} k reads/s, gpu results on the sample "
             f"bit-exact: {cb['bit_exact']}")
 
