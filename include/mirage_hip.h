@@ -156,6 +156,65 @@ mh_status mh_index_get_info(const mh_index *idx, mh_index_info *out);
 mh_status mh_index_bucket(mh_handle *h, const mh_index *idx, uint32_t hash, uint32_t *h_refs,
                           uint32_t *h_pos, uint64_t cap, int64_t *count);
 
+/* ---- .mhm hashmap files (version 4) -------------------------------------- */
+
+/* An opened .mhm file, memory-mapped on the host.  Replaces the reference's
+ * Hashmap::ReadHashmapFile / LoadHashMapFromHashmapDB
+ * (MirAge/Hashmap.cpp:329-580) and the bucket decode of
+ * Analyser.cpp:1596-1720.  The functions below that take no handle run on
+ * the host only; their error text is mh_last_error(NULL) (per thread). */
+typedef struct mh_mhm mh_mhm;
+
+typedef struct {
+  uint64_t n_buckets;       /* TOC entries (Hashmap.cpp:428-430) */
+  uint64_t file_bytes;
+  uint64_t record_bytes;    /* bucket records: first TOC position to end of file */
+  uint64_t ref_bytes;       /* reference sequence bytes */
+  uint64_t header_bytes;    /* reference header bytes */
+  uint32_t n_refs;
+  int32_t threshold;        /* T the file was built with (Hashmap.cpp:456) */
+  uint32_t hash_bytes;      /* HASH_Supported_Bytes: 4 (32-bit build) or 8 */
+  uint32_t streampos_bytes; /* sizeof(HASH_STREAMPOS) of the writing build */
+} mh_mhm_info;
+
+/* Opens and validates the file's header and tables of contents (the
+ * indicator string, version 4, hash and position widths, ascending 28-bit
+ * hashes, in-file extents).  A file that fails a check returns MH_ERR_ARG. */
+mh_status mh_mhm_open(const char *path, mh_mhm **out);
+mh_status mh_mhm_close(mh_mhm *m);
+mh_status mh_mhm_get_info(const mh_mhm *m, mh_mhm_info *out);
+/* Copies the reference sequences as stored (h_off: n_refs+1 offsets), the
+ * headers as stored ('>' name NUL, Hashmap.cpp:2176-2240) and the taxonomy
+ * ids (Hashmap.cpp:552-558).  Any output may be NULL. */
+mh_status mh_mhm_references(const mh_mhm *m, uint8_t *h_bytes, uint64_t *h_off,
+                            uint8_t *h_headers, uint64_t *h_header_off, int32_t *h_taxids);
+/* Decodes the record of bucket `hash` on the host, entries in file order
+ * (ascending position).  *count receives its size, or -1 when the hash is
+ * absent; with MH_INDEX_LAST_BUCKET_EMPTY the last TOC entry decodes empty,
+ * as the reference loader leaves it (Hashmap.cpp:525-538). */
+mh_status mh_mhm_bucket(const mh_mhm *m, uint32_t hash, uint32_t flags, uint32_t *h_refs,
+                        uint32_t *h_pos, uint64_t cap, int64_t *count);
+/* Builds the device index from an opened file: the records are copied to
+ * HBM as stored and decoded there.  threshold 0 takes the file's; a smaller
+ * one is refused as the reference refuses it (Hashmap.cpp:457-463), a
+ * larger one is used for the queries (:464-469).  Every decoded entry is
+ * checked (reference id below n_refs, position inside its reference); a
+ * record that fails returns MH_ERR_ARG naming the bucket. */
+mh_status mh_index_from_mhm(mh_handle *h, const mh_mhm *m, int32_t threshold, uint32_t flags,
+                            mh_index **out);
+/* Writes the index as a .mhm version 4 file, as SaveHashmapTableV3 does
+ * (MirAge/Hashmap.cpp:1701-2540): buckets in ascending hash order, each
+ * sorted by position (ties by reference id), then the sequences (uppercased,
+ * as indexed), the headers and the taxonomy ids.  h_headers/h_header_off:
+ * per reference the bytes MirAge stores ('>' name NUL); NULL writes '>' id
+ * NUL.  h_taxids NULL writes -1.  hash_bytes and streampos_bytes are 4 or 8
+ * (0 selects 4: a 32-bit build).  The index must hold every bucket (built
+ * without MH_INDEX_LAST_BUCKET_EMPTY), else MH_ERR_STATE. */
+mh_status mh_index_save_mhm(mh_handle *h, const mh_index *idx, const char *path,
+                            const uint8_t *h_headers, const uint64_t *h_header_off,
+                            const int32_t *h_taxids, uint32_t hash_bytes,
+                            uint32_t streampos_bytes);
+
 /* ---- classification ---------------------------------------------------- */
 
 /* Full path on device-resident reads: d_reads bytes, d_read_off n_reads+1

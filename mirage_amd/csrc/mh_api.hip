@@ -12,8 +12,12 @@
 
 namespace mh {
 
+// Error text of calls made without a handle, per calling thread.
+static thread_local std::string t_last_error;
+
 mh_status set_error(mh_handle *h, mh_status st, const std::string &msg) {
   if (h) h->last_error = msg;
+  else t_last_error = msg;
   return st;
 }
 
@@ -171,7 +175,9 @@ mh_status mh_close(mh_handle *h) {
   return MH_OK;
 }
 
-const char *mh_last_error(const mh_handle *h) { return h ? h->last_error.c_str() : "null handle"; }
+const char *mh_last_error(const mh_handle *h) {
+  return h ? h->last_error.c_str() : t_last_error.c_str();
+}
 
 void *mh_stream(mh_handle *h) { return h ? (void *)h->stream : nullptr; }
 

@@ -8,7 +8,10 @@
 // (ConvertHashesFromThreadToHashmap, :745-822); for T < 20 buckets above
 // min(20, sizes[floor(0.3 * #buckets)]) are dropped (SaveHashmapTableV3,
 // :1759-1833); the highest-hash bucket can be kept present but empty, as the
-// .mhm loader leaves it (:540-554).
+// .mhm loader leaves it (:540-554).  Windows dropped after a reference's
+// cutoff still count in the size ConvertHashesFromThreadToHashmap converts
+// (:1233-1239, :1252); the unfilled tail of its buffer is value-initialised
+// (:1026), so bucket 0 holds one (ref 0, pos 0) entry per dropped window.
 //
 // Layout in HBM: dir[2^28 + 1] u64 offsets (direct-indexed by hash) and
 // ent[] (u32 ref, u32 pos) pairs, plus the uppercased reference bases.
@@ -89,6 +92,16 @@ __global__ void k_size_hist(const uint32_t *cnt, uint64_t n, unsigned long long 
   }
   __syncthreads();
   if (threadIdx.x < 22 && loc[threadIdx.x]) atomicAdd(&hist[threadIdx.x], loc[threadIdx.x]);
+}
+
+// Adds the dropped-window entries to bucket 0; flags a count past 32 bits.
+__global__ void k_add_phantom(uint32_t *cnt, unsigned long long phantom,
+                              unsigned long long *flag) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const unsigned long long v = (unsigned long long)cnt[0] + phantom;
+    if (v > 0xFFFFFFFFull) *flag = 1;
+    else cnt[0] = (uint32_t)v;
+  }
 }
 
 __global__ void k_drop_large(uint32_t *cnt, uint64_t n, uint32_t thr) {
@@ -195,7 +208,8 @@ mh_status build_index(mh_handle *h, const uint8_t *h_ref_bytes, const uint64_t *
   }
   unsigned long long *d_cut = nullptr;
   uint32_t *d_cnt = nullptr;
-  unsigned long long *d_misc = nullptr;  // [0..21] hist, [22] max hash + 1, [23] nonempty
+  unsigned long long *d_misc = nullptr;  // [0..21] hist, [22] max hash + 1, [23] nonempty,
+                                        // [24] bucket-0 count overflow
   auto cleanup_tmp = [&]() {
     if (d_cut) hipFree(d_cut);
     if (d_cnt) hipFree(d_cnt);
@@ -213,9 +227,18 @@ mh_status build_index(mh_handle *h, const uint8_t *h_ref_bytes, const uint64_t *
   std::vector<uint8_t> nnn(k, 'N');
   const uint32_t h_nnn = host_hash28(nnn.data(), k);
   const unsigned G = grid_for(total, 256);
+  uint64_t phantom = 0;
   if (total) {
     k_cutoff<<<G, 256, 0, st>>>(ix->d_bytes, ix->d_ref_off, n_refs, total, k, h_nnn, d_cut);
+    std::vector<unsigned long long> cut(n_refs);
+    hipMemcpyAsync(cut.data(), d_cut, n_refs * sizeof(uint64_t), hipMemcpyDeviceToHost, st);
+    if ((stt = check(hipStreamSynchronize(st), "cutoff"))) {
+      cleanup_tmp();
+      return fail(stt);
+    }
+    for (uint32_t r = 0; r < n_refs; r++) phantom += stop[r] - cut[r];
     k_count<<<G, 256, 0, st>>>(ix->d_bytes, ix->d_ref_off, n_refs, total, k, d_cut, d_cnt);
+    if (phantom) k_add_phantom<<<1, 64, 0, st>>>(d_cnt, phantom, d_misc + 24);
   }
   if (threshold < 20) {
     k_size_hist<<<grid_for(kDirSize, 256, 4096), 256, 0, st>>>(d_cnt, kDirSize, d_misc);
@@ -247,11 +270,15 @@ mh_status build_index(mh_handle *h, const uint8_t *h_ref_bytes, const uint64_t *
   k_max_hash<<<grid_for(kDirSize, 256, 4096), 256, 0, st>>>(d_cnt, kDirSize,
                                                           (unsigned int *)(d_misc + 22),
                                                           d_misc + 23);
-  unsigned long long mx[2];
+  unsigned long long mx[3];
   hipMemcpyAsync(mx, d_misc + 22, sizeof(mx), hipMemcpyDeviceToHost, st);
   if ((stt = check(hipStreamSynchronize(st), "max hash"))) {
     cleanup_tmp();
     return fail(stt);
+  }
+  if (mx[2]) {
+    cleanup_tmp();
+    return fail(set_error(h, MH_ERR_RANGE, "mh_index_build: bucket 0 would exceed 2^32 entries"));
   }
   const uint32_t mxh = (uint32_t)(mx[0] & 0xFFFFFFFFu);
   ix->n_buckets = mx[1];
@@ -273,8 +300,9 @@ mh_status build_index(mh_handle *h, const uint8_t *h_ref_bytes, const uint64_t *
     hipStreamSynchronize(st);
     hipFree(d_tmp);
   }
-  uint64_t n_ent = 0;
+  uint64_t n_ent = 0, bucket0 = 0;
   hipMemcpyAsync(&n_ent, ix->d_dir + kDirSize, sizeof(uint64_t), hipMemcpyDeviceToHost, st);
+  hipMemcpyAsync(&bucket0, ix->d_dir + 1, sizeof(uint64_t), hipMemcpyDeviceToHost, st);
   if ((stt = check(hipStreamSynchronize(st), "scan"))) {
     cleanup_tmp();
     return fail(stt);
@@ -285,6 +313,9 @@ mh_status build_index(mh_handle *h, const uint8_t *h_ref_bytes, const uint64_t *
     return fail(stt);
   }
   hipMemsetAsync(d_cnt, 0, (kDirSize + 1) * sizeof(uint32_t), st);
+  // The scatter fills bucket 0's windows first; the (0, 0) entries after them
+  // are zero bytes.
+  if (phantom && bucket0) hipMemsetAsync(ix->d_ent, 0, bucket0 * sizeof(uint2), st);
   if (total)
     k_scatter<<<G, 256, 0, st>>>(ix->d_bytes, ix->d_ref_off, n_refs, total, k, d_cut, ix->d_dir,
                                  d_cnt, ix->d_ent);

@@ -40,6 +40,14 @@ class IndexInfo(ctypes.Structure):
                 ("pad", ctypes.c_uint32)]
 
 
+class MhmInfo(ctypes.Structure):
+    _fields_ = [("n_buckets", ctypes.c_uint64), ("file_bytes", ctypes.c_uint64),
+                ("record_bytes", ctypes.c_uint64), ("ref_bytes", ctypes.c_uint64),
+                ("header_bytes", ctypes.c_uint64), ("n_refs", ctypes.c_uint32),
+                ("threshold", ctypes.c_int32), ("hash_bytes", ctypes.c_uint32),
+                ("streampos_bytes", ctypes.c_uint32)]
+
+
 class Counters(ctypes.Structure):
     _fields_ = [(f, ctypes.c_uint64) for f in COUNTER_FIELDS]
 
@@ -72,7 +80,9 @@ EXPORTS = ("mh_open", "mh_close", "mh_last_error", "mh_stream", "mh_max_read_len
            "mh_max_batch_reads", "mh_index_build", "mh_index_free", "mh_index_get_info",
            "mh_index_bucket", "mh_classify", "mh_classify_host", "mh_get_counters", "mh_get_timing",
            "mh_score_seeds", "mh_set_tiles", "mh_get_tiles", "mh_autotune", "mh_server_start",
-           "mh_server_submit", "mh_server_wait", "mh_server_stop")
+           "mh_server_submit", "mh_server_wait", "mh_server_stop", "mh_mhm_open", "mh_mhm_close",
+           "mh_mhm_get_info", "mh_mhm_references", "mh_mhm_bucket", "mh_index_from_mhm",
+           "mh_index_save_mhm")
 
 _lib = None
 
@@ -117,6 +127,13 @@ def load():
         "mh_server_wait": (i32, [P, u64, ctypes.c_double, P, P, u64, ctypes.POINTER(u64),
                                  ctypes.POINTER(RequestInfo)]),
         "mh_server_stop": (i32, [P]),
+        "mh_mhm_open": (i32, [ctypes.c_char_p, ctypes.POINTER(P)]),
+        "mh_mhm_close": (i32, [P]),
+        "mh_mhm_get_info": (i32, [P, ctypes.POINTER(MhmInfo)]),
+        "mh_mhm_references": (i32, [P, P, P, P, P, P]),
+        "mh_mhm_bucket": (i32, [P, u32, u32, P, P, u64, ctypes.POINTER(ctypes.c_int64)]),
+        "mh_index_from_mhm": (i32, [P, P, i32, u32, ctypes.POINTER(P)]),
+        "mh_index_save_mhm": (i32, [P, P, ctypes.c_char_p, P, P, P, u32, u32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name, None)
@@ -173,6 +190,21 @@ class Handle:
     def build_index(self, ref_bytes, ref_off, threshold, last_bucket_empty=True):
         return DeviceIndex(self, ref_bytes, ref_off, threshold, last_bucket_empty)
 
+    def load_mhm(self, mhm, threshold=0, last_bucket_empty=True):
+        """Device index from a .mhm file (a path or an open MhmFile),
+        decoded in HBM (mh_index_from_mhm)."""
+        f = mhm if isinstance(mhm, MhmFile) else MhmFile(mhm)
+        try:
+            ix = ctypes.c_void_p()
+            flags = MH_INDEX_LAST_BUCKET_EMPTY if last_bucket_empty else 0
+            self.check(self.lib.mh_index_from_mhm(self.h, f.m, threshold, flags, ctypes.byref(ix)),
+                       "mh_index_from_mhm")
+            info = f.info()
+        finally:
+            if f is not mhm:
+                f.close()
+        return DeviceIndex.wrap(self, ix, threshold or info["threshold"], info["n_refs"])
+
     def set_tiles(self, tiles):
         arr = (Tile * len(tiles))(*[Tile(*t) for t in tiles])
         self.check(self.lib.mh_set_tiles(self.h, ctypes.cast(arr, ctypes.c_void_p), len(tiles)),
@@ -220,6 +252,26 @@ class DeviceIndex:
         self.ix = ix
         self.threshold = threshold
         self.n_refs = len(ro) - 1
+
+    @classmethod
+    def wrap(cls, handle, ix, threshold, n_refs):
+        self = cls.__new__(cls)
+        self.handle, self.ix, self.threshold, self.n_refs = handle, ix, threshold, n_refs
+        return self
+
+    def save_mhm(self, path, headers=None, taxids=None, hash_bytes=4, streampos_bytes=4):
+        """Writes the index as a .mhm v4 file (mh_index_save_mhm).  headers:
+        per reference the bytes MirAge stores ('>' name NUL), or None."""
+        hb = ho = tx = None
+        if headers is not None:
+            hb = np.frombuffer(b"".join(headers) or b"\0", np.uint8)
+            ho = np.zeros(len(headers) + 1, np.uint64)
+            ho[1:] = np.cumsum([len(x) for x in headers])
+        if taxids is not None:
+            tx = np.ascontiguousarray(taxids, dtype=np.int32)
+        self.handle.check(self.handle.lib.mh_index_save_mhm(
+            self.handle.h, self.ix, os.fsencode(path), _np_ptr(hb), _np_ptr(ho), _np_ptr(tx),
+            hash_bytes, streampos_bytes), "mh_index_save_mhm")
 
     def free(self):
         if self.ix:
@@ -292,6 +344,80 @@ class DeviceIndex:
                              ctypes.byref(cn) if wait else None)
         self.handle.check(st, "mh_classify")
         return (tn.value, cn.value) if wait else None
+
+
+def last_error():
+    """Error text of the calling thread's last call made without a handle."""
+    return load().mh_last_error(None).decode()
+
+
+class MhmFile:
+    """An opened .mhm v4 hashmap file (mh_mhm_open); host-side only."""
+
+    def __init__(self, path):
+        self.lib = load()
+        m = ctypes.c_void_p()
+        st = self.lib.mh_mhm_open(os.fsencode(path), ctypes.byref(m))
+        if st != MH_OK:
+            raise MirageError(st, last_error())
+        self.m = m
+
+    def close(self):
+        if self.m:
+            self.lib.mh_mhm_close(self.m)
+            self.m = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def info(self):
+        i = MhmInfo()
+        st = self.lib.mh_mhm_get_info(self.m, ctypes.byref(i))
+        if st != MH_OK:
+            raise MirageError(st, last_error())
+        return {f: getattr(i, f) for f, _ in MhmInfo._fields_}
+
+    def references(self):
+        """(sequence bytes, n_refs+1 offsets, header byte strings, taxonomy ids)."""
+        i = self.info()
+        n = i["n_refs"]
+        rb = np.zeros(max(i["ref_bytes"], 1), np.uint8)
+        ro = np.zeros(n + 1, np.uint64)
+        hb = np.zeros(max(i["header_bytes"], 1), np.uint8)
+        ho = np.zeros(n + 1, np.uint64)
+        tx = np.zeros(max(n, 1), np.int32)
+        st = self.lib.mh_mhm_references(self.m, _np_ptr(rb), _np_ptr(ro), _np_ptr(hb), _np_ptr(ho),
+                                        _np_ptr(tx))
+        if st != MH_OK:
+            raise MirageError(st, last_error())
+        heads = [hb[int(ho[r]):int(ho[r + 1])].tobytes() for r in range(n)]
+        return rb[:i["ref_bytes"]], ro, heads, tx[:n]
+
+    def bucket(self, h, last_bucket_empty=True):
+        """Entries (ref, pos) of bucket h in file order, or None when absent."""
+        n = ctypes.c_int64()
+        fl = MH_INDEX_LAST_BUCKET_EMPTY if last_bucket_empty else 0
+        st = self.lib.mh_mhm_bucket(self.m, h, fl, None, None, 0, ctypes.byref(n))
+        if st != MH_OK:
+            raise MirageError(st, last_error())
+        if n.value < 0:
+            return None
+        refs = np.zeros(max(n.value, 1), np.uint32)
+        pos = np.zeros(max(n.value, 1), np.uint32)
+        st = self.lib.mh_mhm_bucket(self.m, h, fl, _np_ptr(refs), _np_ptr(pos), n.value,
+                                    ctypes.byref(n))
+        if st != MH_OK:
+            raise MirageError(st, last_error())
+        return list(zip(refs[:n.value].tolist(), pos[:n.value].tolist()))
 
 
 class Server:

@@ -11,7 +11,10 @@
  *                    through oracle/ref_harness when oracle/_ref is built.
  *   index (a2,a3)    brute-force enumeration on tiny references; the
  *                    "N"-window cutoff; the fragment rule at T < 20;
- *                    entry-count invariant.
+ *                    entry-count invariant; bucket-for-bucket equality with
+ *                    .mhm files the reference wrote (tests/golden).
+ *   .mhm (f1)        oracle/mhm.py decodes those files and writes them back
+ *                    byte for byte.
  *   candidates       brute-force transcription on tiny inputs plus
  *   (a5-a9)          end-to-end golden outputs written by the reference's own
  *                    binary (JSON fixtures and their generator in tests/golden).
@@ -28,13 +31,9 @@
  *     the probe loop at :1474-1476); such reads get no candidates here.
  *   - a reference of length n <= k hashes past its end
  *     (Hashmap.cpp:2839-2841); it contributes no index entries here.
- *   - bucket positions are stored exactly.  The reference's .mhm writer
- *     delta-encodes positions against the first entry of an unsorted bucket
- *     (Hashmap.cpp:1629-1642 with SortAndCorrectPairVector, :1561, never
- *     called), which corrupts some positions of buckets with three or more
- *     entries, depending on an entry order that follows thread completion
- *     (Hashmap.cpp:2906-2911).  That order is not a function of the input, so
- *     it is not reproduced.
+ *
+ * The index equals, bucket for bucket, what the reference writes to its .mhm
+ * file and loads back (checked against files the reference wrote).
  */
 #include "mirage_oracle.h"
 
@@ -155,7 +154,10 @@ int64_t find_key(const mo_index &ix, uint32_t h) {
  * positions i in [0, n-k) when k < n.  A window whose hash equals H("N"*k) is
  * dropped without advancing the hashed pointer (:2882-2895), so every later
  * window of that reference hashes the same bytes and is dropped as well: the
- * reference is indexed only up to its first such window.
+ * reference is indexed only up to its first such window.  The dropped
+ * windows still count in the size passed to ConvertHashesFromThreadToHashmap
+ * (:1233-1239, :1252), so bucket 0 gains one (ref 0, pos 0) entry per
+ * dropped window (the value-initialised tail of the build buffer, :1026).
  * ConvertHashesFromThreadToHashmap (:745-822) groups by hash; the fragment
  * rule of SaveHashmapTableV3 (:1759-1809, :1831-1833) drops buckets larger
  * than min(20, sizes[floor(0.30 * #buckets)]) when T < 20.  With
@@ -196,7 +198,16 @@ extern "C" mo_index *mo_index_build(const uint8_t *ref_bytes,
     ix->cutoff[r] = i;
   });
   for (uint32_t r = 0; r < n_refs; r++) nwin[r + 1] = nwin[r] + ix->cutoff[r];
-  const uint64_t n_entries = nwin[n_refs];
+  // Windows dropped after a reference's cutoff still count towards the size
+  // handed to ConvertHashesFromThreadToHashmap (Hashmap.cpp:1233-1239,
+  // :1252), whose tail past the filled entries holds value-initialised
+  // records (resize at :1026): one (ref 0, pos 0) entry of hash 0 each.
+  uint64_t phantom = 0;
+  for (uint32_t r = 0; r < n_refs; r++) {
+    const uint64_t n = ix->ref_off[r + 1] - ix->ref_off[r];
+    if (n > (uint64_t)k) phantom += (n - k) - ix->cutoff[r];
+  }
+  const uint64_t n_entries = nwin[n_refs] + phantom;
 
   // Partition by the top 8 hash bits, then sort each partition.
   constexpr int kParts = 256;
@@ -208,6 +219,7 @@ extern "C" mo_index *mo_index_build(const uint8_t *ref_bytes,
     for (uint64_t i = 0; i < ix->cutoff[r]; i++)
       out[i] = Entry{mo_hash28(p + i, k), (uint32_t)r, (uint32_t)i};
   });
+  for (uint64_t i = nwin[n_refs]; i < n_entries; i++) tmp[i] = Entry{0u, 0u, 0u};
   std::vector<uint64_t> part_cnt(kParts + 1, 0);
   for (uint64_t i = 0; i < n_entries; i++) part_cnt[(tmp[i].hash >> 20) + 1]++;
   for (int b = 0; b < kParts; b++) part_cnt[b + 1] += part_cnt[b];

@@ -35,3 +35,26 @@ def expected_lines(fx, n_reads):
         e = fx["expected"].get(str(i))
         out.append(None if e is None else (e[0], tuple(e[1])))
     return out
+
+
+def mhm_names():
+    """Datasets whose reference-written .mhm file is a fixture."""
+    return sorted(os.path.basename(p)[4:-7] for p in glob.glob(os.path.join(GOLDEN, "ref_*.mhm.gz")))
+
+
+def mhm_bytes(name):
+    with gzip.open(os.path.join(GOLDEN, f"ref_{name}.mhm.gz"), "rb") as f:
+        return f.read()
+
+
+def mhm_file(name, directory):
+    """Writes the decompressed .mhm fixture under directory; returns its path."""
+    path = os.path.join(str(directory), f"{name}.mhm")
+    with open(path, "wb") as f:
+        f.write(mhm_bytes(name))
+    return path
+
+
+def headers(n):
+    """Reference headers as MirAge keeps them for the fixtures' FASTA names."""
+    return [b">r%d\0" % i for i in range(n)]

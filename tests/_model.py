@@ -41,6 +41,7 @@ class Model:
         k = self.k
         nnn = hash28(b"N" * k)
         buckets = {}
+        phantom = 0
         for rid, r in enumerate(self.refs):
             n = len(r)
             if n <= k:
@@ -48,10 +49,16 @@ class Model:
             for i in range(n - k):
                 h = hash28(r[i:i + k])
                 if h == nnn and mutation != "no_nnn_cutoff":
-                    break  # the hashed pointer stops advancing (Hashmap.cpp:2882-2895)
+                    # the hashed pointer stops advancing (Hashmap.cpp:2882-2895);
+                    # the windows left still count in the converted size
+                    # (:1233-1239, :1252) as value-initialised (0, 0) entries
+                    phantom += (n - k) - i
+                    break
                 if h == nnn:
                     continue
                 buckets.setdefault(h, []).append((rid, i))
+        if phantom:
+            buckets.setdefault(0, []).extend([(0, 0)] * phantom)
         if T < 20 and buckets:
             sizes = sorted(len(v) for v in buckets.values())
             thr = min(20, sizes[int((len(sizes) / 100.0) * 30)])

@@ -9,7 +9,9 @@ Each fixture is seeded synthetic data (references and reads as strings) plus
 what the reference printed for it: per read id, the relative score string and
 the top-tie reference ids (results_Raw.log, Analyser.cpp:228-376).  Reads
 without a line are unclassified.  The binary is run twice per dataset; a
-fixture is written only when both runs agree.
+fixture is written only when both runs agree (results and the .mhm file).
+For the datasets in MHM_FIXTURES the reference's .mhm hashmap file is kept
+as well (ref_<name>.mhm.gz).
 """
 import gzip
 import json
@@ -84,6 +86,21 @@ def datasets():
            D.ReadSpec(n=250, lengths=L(mode=2, a=40, b=400), p_exact=0.2, p_random=0.1,
                       sub_rate=0.01, rc_frac=0.5),
            107, None)
+    yield ("t18_tiny_mixed", 18,
+           D.RefSpec(n=6, lengths=L(mode=2, a=300, b=2500), gc_lo=0.3, gc_hi=0.7),
+           D.ReadSpec(n=200, lengths=L(mode=2, a=40, b=300), p_exact=0.2, p_random=0.1,
+                      sub_rate=0.01, rc_frac=0.5),
+           112, None)
+    yield ("t30_tiny_nruns", 30,
+           D.RefSpec(n=6, lengths=L(mode=0, a=2000)),
+           D.ReadSpec(n=200, lengths=L(mode=0, a=150), sub_rate=0.01, rc_frac=0.5),
+           114, "n_runs")
+
+
+# Datasets whose .mhm file (the reference's own hashmap database, written by
+# SaveHashmapTableV3 and loaded back before classification) is kept as a
+# fixture too: ref_<name>.mhm.gz.
+MHM_FIXTURES = ("t22_tiny_mixed", "t18_tiny_mixed", "t30_tiny_nruns")
 
 
 COMP = np.zeros(256, np.uint8)
@@ -232,7 +249,9 @@ def run_reference(workdir, threshold):
             if fn == "results_Raw.log":
                 outs.append(os.path.join(root, fn))
     assert len(outs) == 1, outs
-    res = {}
+    mhm = [os.path.join(os.path.dirname(outs[0]), "hashmap_reference.mhm")]
+    assert os.path.exists(mhm[0]), mhm
+    res = {"_mhm": mhm[0]}
     with open(outs[0]) as f:
         next(f)
         for line in f:
@@ -267,9 +286,13 @@ def main():
         write_fasta(os.path.join(base, "query.fasta"), "q", qb, qo)
         r1 = run_reference(os.path.join(base, "a"), T)
         r2 = run_reference(os.path.join(base, "b"), T)
-        if r1 != r2:
+        mhm = [open(r.pop("_mhm"), "rb").read() for r in (r1, r2)]
+        if r1 != r2 or mhm[0] != mhm[1]:
             print(f"{name}: reference runs disagree, fixture not written")
             continue
+        if name in MHM_FIXTURES:
+            with gzip.GzipFile(os.path.join(OUT_DIR, f"ref_{name}.mhm.gz"), "wb", mtime=0) as f:
+                f.write(mhm[0])
         fx = {
             "name": name,
             "threshold": T,
@@ -279,8 +302,8 @@ def main():
                       for i in range(len(qo) - 1)],
             "expected": {str(k): v for k, v in sorted(r1.items())},
         }
-        with gzip.open(os.path.join(OUT_DIR, f"ref_{name}.json.gz"), "wt") as f:
-            json.dump(fx, f)
+        with gzip.GzipFile(os.path.join(OUT_DIR, f"ref_{name}.json.gz"), "wb", mtime=0) as f:
+            f.write(json.dumps(fx).encode())
         print(f"{name}: {len(r1)} classified of {len(qo) - 1}")
 
 

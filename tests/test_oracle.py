@@ -98,7 +98,12 @@ def test_index_invariants():
     cut = [ix.cutoff(r) for r in range(len(refs))]
     assert cut[0] == 50 - k and cut[1] == 300 - k and cut[3] == 17 - k
     assert cut[2] == 600 - k + 1 or cut[2] <= 600  # first all-N window ends the reference
-    assert ix.num_entries == sum(cut)
+    # windows after the cut of reference 2 come back as (0, 0) entries of
+    # bucket 0 (the reference converts its reserved size, Hashmap.cpp:1252)
+    phantom = (1000 - k) - cut[2]
+    assert phantom > 0
+    assert ix.num_entries == sum(cut) + phantom
+    assert (ix.bucket(0) or []).count((0, 0)) == phantom
     # brute force: every indexed window is in its bucket
     for r, seq in enumerate(refs):
         for i in range(0, cut[r], 37):
