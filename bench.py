@@ -144,6 +144,66 @@ def serving_leg(native, D, scale, n_requests, loads, log):
     return out
 
 
+def mhm_leg(torch, native, handle, ix, info, rb, ro, threshold, d_reads, d_off, n, out, top,
+            log):
+    """Writes the workload's index as a .mhm v4 file (host memory file
+    system when there is one) and loads it back onto the device three times:
+    end-to-end load time from the mapped file, including the copy to HBM and
+    the decode kernels.  The loaded index must equal the built one and
+    classify the reads identically."""
+    import shutil
+    import tempfile
+    d = "/dev/shm" if os.path.isdir("/dev/shm") else tempfile.gettempdir()
+    need = 24 * info["n_entries"] + (2 << 30)
+    if shutil.disk_usage(d).free < need:
+        log(f"mhm: skipped, {d} has less than {need / 2**30:.1f} GiB free")
+        return {"skipped": f"{d} too small"}
+    path = os.path.join(d, f"mh_bench_{os.getpid()}.mhm")
+    full = handle.build_index(rb, ro, threshold, last_bucket_empty=False)
+    try:
+        t = time.perf_counter()
+        # 8-byte hashes and stream positions, the 64-bit build's format: the
+        # file is larger than 4 GiB
+        full.save_mhm(path, hash_bytes=8, streampos_bytes=8)
+        save_s = time.perf_counter() - t
+    finally:
+        full.free()
+    size = os.path.getsize(path)
+    log(f"mhm: wrote {size / 2**30:.2f} GiB in {save_s:.2f} s")
+    loads, same_index, same_results = [], True, None
+    keys = ("n_entries", "n_buckets", "empty_hash", "n_refs", "threshold")
+    try:
+        for rep in range(3):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            lx = handle.load_mhm(path)
+            loads.append(time.perf_counter() - t)
+            li = lx.info()
+            same_index &= all(li[k] == info[k] for k in keys)
+            if rep == 0:
+                lout = torch.empty(n * native.READ_RESULT_DTYPE.itemsize, dtype=torch.uint8,
+                                   device="cuda")
+                ltop = torch.empty(len(top) * 4, dtype=torch.uint8, device="cuda")
+                lx.classify_device(d_reads, d_off, n, lout, ltop, len(top),
+                                   flags=native.MH_CLASSIFY_BATCHED)
+                o2 = lout.cpu().numpy().view(native.READ_RESULT_DTYPE)
+                t2 = ltop.cpu().numpy().view(np.uint32)
+                same_results = bool(
+                    np.array_equal(o2["highest"].view(np.uint32), out["highest"].view(np.uint32))
+                    and native.tie_sets(o2, t2) == native.tie_sets(out, top))
+                del lout, ltop
+            lx.free()
+    finally:
+        os.remove(path)
+    best = min(loads)
+    log(f"mhm: load {best:.3f} s ({size / best / 1e9:.2f} GB/s of file), index same "
+        f"{same_index}, results same {same_results}")
+    return {"file_bytes": size, "save_s": round(save_s, 3), "load_s": round(best, 4),
+            "load_s_all": [round(x, 4) for x in loads],
+            "load_file_GBps": round(size / best / 1e9, 3), "dir": d,
+            "index_same_as_build": bool(same_index), "results_same": same_results}
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -215,6 +275,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--mixed-reads", type=int, default=2_000_000)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-mhm", action="store_true")
     ap.add_argument("--serve-scale", type=float, default=1.0)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -330,6 +391,11 @@ def main():
             f"M reads/s, x{ps / bs:.2f}")
         del md_reads, md_off
 
+    mhm = None
+    if not args.no_mhm:
+        mhm = mhm_leg(torch, native, handle, ix, info, rb, ro, w.threshold, d_reads, d_off, n,
+                      out, top, log)
+
     # Roofline of the dominant kernel: the tile kernel of the shortest length
     # bucket (16-lane teams, reads up to 200 bp), which holds every read of
     # this workload.  Algorithmic bytes per launch: a 16-byte directory pair
@@ -398,6 +464,7 @@ def main():
         "preprobe": pre,
         "plain": plain,
         "mixed_length": mixed,
+        "mhm_load": mhm,
         "served": serve,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -231,9 +231,11 @@ mh_status mh_index_bucket(mh_handle *h, const mh_index *idx, uint32_t hash, uint
     *count = (hash == idx->empty_hash) ? 0 : -1;
     return MH_OK;
   }
-  const uint64_t n = b[1] - b[0];
-  std::vector<uint2> e(n);
-  MH_HIP(h, hipMemcpy(e.data(), idx->d_ent + b[0], n * sizeof(uint2), hipMemcpyDeviceToHost));
+  const uint64_t stored = b[1] - b[0];
+  const uint64_t n = stored + (hash == 0 ? idx->phantom : 0);  // identical (0, 0) copies
+  std::vector<uint2> e(n, make_uint2(0, 0));
+  MH_HIP(h, hipMemcpy(e.data(), idx->d_ent + b[0], stored * sizeof(uint2),
+                      hipMemcpyDeviceToHost));
   std::sort(e.begin(), e.end(), [](const uint2 &a, const uint2 &c) {
     return a.x < c.x || (a.x == c.x && a.y < c.y);
   });

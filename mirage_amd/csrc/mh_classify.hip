@@ -493,7 +493,9 @@ __device__ __forceinline__ void set_bits(uint32_t *bm, uint32_t a, uint32_t len)
 }
 
 // Scores one reference from its seeds: every wave takes seeds in turn.
-// get_seed(i, sref, sq, lref, lq) supplies seed i.  Returns on all threads.
+// get_seed(i, sref, sq, lref, lq, wt) supplies seed i and its number of
+// identical copies wt, whose sums count wt times (the row bitmap once).
+// Returns on all threads.
 template <class TM, class GetSeed>
 __device__ void score_ref(const uint8_t *q, uint32_t m, const uint8_t *rb, uint64_t ref_len,
                           uint32_t n_seeds, int T, bool ultrafast, const Ws &ws,
@@ -510,16 +512,16 @@ __device__ void score_ref(const uint8_t *q, uint32_t m, const uint8_t *rb, uint6
   TM::sync();
   unsigned long long sr = 0, sc = 0, ncmp = 0;
   for (uint32_t i = wid; i < n_seeds; i += NW) {
-    uint32_t sref, sq, lref, lq;
-    get_seed(i, sref, sq, lref, lq);
+    uint32_t sref, sq, lref, lq, wt;  // wt: copies of this seed
+    get_seed(i, sref, sq, lref, lq, wt);
     const uint32_t rs = sq, re = lq;
     if (rs > re) continue;  // MatrixCPU.cpp:57-58
     uint32_t tot, h, cmp;
     extend_seed<TM>(q, m, rb, ref_len, sref, rs, lref, re, T, ultrafast, tot, h, cmp);
-    ncmp += cmp;
+    ncmp += (unsigned long long)wt * cmp;
     if (tot >= (uint32_t)T) {  // MatrixCPU.cpp:178-184
-      sc += tot;
-      sr += (unsigned long long)tot * (tot + 1) / 2;
+      sc += (unsigned long long)wt * tot;
+      sr += (unsigned long long)wt * ((unsigned long long)tot * (tot + 1) / 2);
       set_bits<TM>(ws.bitmap, rs - h, tot);
     }
   }
@@ -1059,7 +1061,7 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
 
   // Probe every s-th window; compact present hashes into the hit list.
   // With probe records, the lookups were made beforehand by k_probe.
-  uint32_t nhits = 0, big = 0;
+  uint32_t nhits = 0, big = 0, ph = 0;
   for (uint32_t base = 0; base < nprobe; base += BS) {
     const uint32_t p = base + tid;
     uint32_t present = 0, cnt = 0;
@@ -1093,7 +1095,10 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
       ws.hit_base[nhits + idx] = cnt;
     }
     nhits += tot;
+    ph += (lo == 0 && cnt > 0) ? 1u : 0u;  // bucket 0, the only one at offset 0
   }
+  // Probes that hit bucket 0 while it holds unstored copies of (0, 0).
+  const uint32_t nph = ix.phantom ? TM::sum(ph, ws.red) : 0u;
   TM::sync();
   uint32_t E = 0;
   for (uint32_t base = 0; base < nhits; base += BS) {
@@ -1116,7 +1121,7 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
   if (tid == 0) {
     ws.stats[kProbes] += nprobe;
     ws.stats[kHits] += nhits;
-    ws.stats[kEntries] += E;
+    ws.stats[kEntries] += E + (unsigned long long)nph * ix.phantom;
   }
   if (nhits == 0 || E == 0) {  // Analyser.cpp:1989-1990, :2012-2013
     write_query<TM>(c, q, res);
@@ -1140,7 +1145,7 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
   uint32_t nP = 0, nranks = 0, nchains = 0, G = 0;
   bool direct = false, sorted = false, small = false;
   if constexpr (BS <= 64) {
-    if (E <= 2 * BS && E <= 64 && nhits <= 64 && m < (1u << 26)) {  // qPos fits 26 bits
+    if (E <= 2 * BS && E <= 64 && nhits <= 64 && m < (1u << 26) && nph == 0) {
       small = true;
       const uint32_t r = small_path<TM>(c, ws, E, nhits, nprobe, nP);
       if (r == kSmallDirect) {
@@ -1152,7 +1157,10 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
       sorted = r == kSmallSorted;
     }
   }
-  if (!small && !(c.flags & MH_CLASSIFY_NO_DIRECT) && nhits == nprobe && nhits >= 3) {
+  // (qPos fits 26 bits in the register path; a strand that hit the copies of
+  // (0, 0) takes the general path, where their seeds are weighted.)
+  if (!small && !(c.flags & MH_CLASSIFY_NO_DIRECT) && nhits == nprobe && nhits >= 3 &&
+      nph == 0) {
     // Direct hit.  Every probe found a bucket, and one reference r* has
     // exactly one entry in each bucket, all on one diagonal.  Then r* has the
     // longest run N = nhits, the cut is N - 1, and if no other reference
@@ -1341,7 +1349,8 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
   const uint32_t *cfl = (const uint32_t *)ws.k1;
   float *rel_of = (float *)ws.t2;
   uint32_t *valid_of = ws.t1;
-  uint32_t seeds_total = 0, seeded = 0;
+  unsigned long long seeds_total = 0;
+  uint32_t seeded = 0;
   for (uint32_t r = 0; r < nranks; r++) {
     if (ws.t3[r] != G) {
       if (tid == 0) valid_of[r] = 0;
@@ -1353,17 +1362,41 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
     const uint64_t ro = ix.ref_off[R];
     const uint64_t ref_len = ix.ref_off[R + 1] - ro;
     const uint8_t *rb = ix.bytes + ro;
-    auto seed = [&](uint32_t i, uint32_t &sref, uint32_t &sq, uint32_t &lref, uint32_t &lq) {
-      const uint32_t cid = c0 + i;
-      const uint32_t f = cfl[2 * cid], l = cfl[2 * cid + 1];
-      sref = (uint32_t)ws.k2[f];
-      sq = ws.v2[f];
-      lref = (uint32_t)ws.k2[l] + (uint32_t)s;  // Analyser.cpp:2465-2467
-      lq = ws.v2[l] + (uint32_t)s;
+    // Reference 0 also gets, per probe j that hit bucket 0, ix.phantom more
+    // one-pair seeds (0, j, s, j + s): the unstored copies of (0, 0, j), each
+    // a chain of its own (Analyser.cpp:2421-2481).
+    const uint32_t nvirt = R == 0 ? nph : 0u;
+    auto seed = [&](uint32_t i, uint32_t &sref, uint32_t &sq, uint32_t &lref, uint32_t &lq,
+                    uint32_t &wt) {
+      if (i < c1 - c0) {
+        const uint32_t cid = c0 + i;
+        const uint32_t f = cfl[2 * cid], l = cfl[2 * cid + 1];
+        sref = (uint32_t)ws.k2[f];
+        sq = ws.v2[f];
+        lref = (uint32_t)ws.k2[l] + (uint32_t)s;  // Analyser.cpp:2465-2467
+        lq = ws.v2[l] + (uint32_t)s;
+        wt = 1;
+        return;
+      }
+      uint32_t v = i - (c1 - c0), j = 0;
+      for (uint32_t t = 0; t < nhits; t++)
+        if (ws.hit_lo[t] == 0 && ws.hit_base[t + 1] > ws.hit_base[t]) {
+          if (v == 0) {
+            j = ws.hit_j[t];
+            break;
+          }
+          v--;
+        }
+      sref = 0;
+      sq = j;
+      lref = (uint32_t)s;
+      lq = j + (uint32_t)s;
+      wt = ix.phantom;
     };
     uint64_t sum_r, sum_c;
     uint32_t hits;
-    score_ref<TM>(ws.q, m, rb, ref_len, c1 - c0, T, ultrafast, ws, seed, sum_r, sum_c, hits);
+    score_ref<TM>(ws.q, m, rb, ref_len, c1 - c0 + nvirt, T, ultrafast, ws, seed, sum_r, sum_c,
+                  hits);
     const float score = matrix_score(sum_r, sum_c, hits, m);
     float rel = -1.0f;
     uint32_t valid = 0, lut_bad = 0;
@@ -1371,7 +1404,7 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
       if (rel_score(c.lut, m, ref_len, score, rel)) valid = 1;
       else lut_bad = 1;
     }
-    seeds_total += c1 - c0;
+    seeds_total += (c1 - c0) + (unsigned long long)nvirt * ix.phantom;
     seeded++;
     if (tid == 0) {
       rel_of[r] = rel;
@@ -1389,7 +1422,7 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
           cd.sum_c = sum_c;
           cd.score = score;
           cd.rel = rel;
-          cd.n_seeds = c1 - c0;
+          cd.n_seeds = (uint32_t)((c1 - c0) + (unsigned long long)nvirt * ix.phantom);
           cd.pad = 0;
           c.out.cands[slot] = cd;
         }
@@ -1830,12 +1863,14 @@ __global__ __launch_bounds__(64) void k_score_seeds(IndexView ix, Luts lut, cons
       const uint64_t s0 = soff[p], s1 = soff[p + 1];
       const uint64_t ro = ix.ref_off[R];
       const uint64_t ref_len = ix.ref_off[R + 1] - ro;
-      auto seed = [&](uint32_t i, uint32_t &sref, uint32_t &sq, uint32_t &lref, uint32_t &lq) {
+      auto seed = [&](uint32_t i, uint32_t &sref, uint32_t &sq, uint32_t &lref, uint32_t &lq,
+                      uint32_t &wt) {
         const uint32_t *sd = seeds + 4 * (s0 + i);
         sref = sd[0];
         sq = sd[1];
         lref = sd[2];
         lq = sd[3];
+        wt = 1;
       };
       if (s1 > s0) {
         score_ref<BlockTeam<64>>(ws.q, m, ix.bytes + ro, ref_len, (uint32_t)(s1 - s0), ix.T,

@@ -12,6 +12,8 @@
 // cutoff still count in the size ConvertHashesFromThreadToHashmap converts
 // (:1233-1239, :1252); the unfilled tail of its buffer is value-initialised
 // (:1026), so bucket 0 holds one (ref 0, pos 0) entry per dropped window.
+// Those identical entries are stored once; the count of the others is kept
+// beside the index (mh_index::phantom) and weighted by the kernels.
 //
 // Layout in HBM: dir[2^28 + 1] u64 offsets (direct-indexed by hash) and
 // ent[] (u32 ref, u32 pos) pairs, plus the uppercased reference bases.
@@ -152,6 +154,8 @@ uint32_t host_hash28(const uint8_t *p, int k) {
   return ((h >> kHashBits) ^ h) & kHashMask;
 }
 
+uint8_t host_upper(uint8_t c) { return (int8_t)c > 96 ? (uint8_t)(c - 32) : c; }
+
 unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 65536) {
   uint64_t g = (n + block - 1) / block;
   return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
@@ -227,7 +231,7 @@ mh_status build_index(mh_handle *h, const uint8_t *h_ref_bytes, const uint64_t *
   std::vector<uint8_t> nnn(k, 'N');
   const uint32_t h_nnn = host_hash28(nnn.data(), k);
   const unsigned G = grid_for(total, 256);
-  uint64_t phantom = 0;
+  uint64_t phantom = 0, cut0 = 0;
   if (total) {
     k_cutoff<<<G, 256, 0, st>>>(ix->d_bytes, ix->d_ref_off, n_refs, total, k, h_nnn, d_cut);
     std::vector<unsigned long long> cut(n_refs);
@@ -237,6 +241,7 @@ mh_status build_index(mh_handle *h, const uint8_t *h_ref_bytes, const uint64_t *
       return fail(stt);
     }
     for (uint32_t r = 0; r < n_refs; r++) phantom += stop[r] - cut[r];
+    if (n_refs) cut0 = cut[0];
     k_count<<<G, 256, 0, st>>>(ix->d_bytes, ix->d_ref_off, n_refs, total, k, d_cut, d_cnt);
     if (phantom) k_add_phantom<<<1, 64, 0, st>>>(d_cnt, phantom, d_misc + 24);
   }
@@ -286,6 +291,31 @@ mh_status build_index(mh_handle *h, const uint8_t *h_ref_bytes, const uint64_t *
     ix->empty_hash = mxh - 1;
     hipMemsetAsync(d_cnt + ix->empty_hash, 0, sizeof(uint32_t), st);
   }
+  // Bucket 0 keeps one (0, 0) entry: a window of reference 0 at position 0
+  // with hash 0, or else one of the dropped-window entries.
+  if (phantom) {
+    uint32_t c0 = 0;
+    hipMemcpyAsync(&c0, d_cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if ((stt = check(hipStreamSynchronize(st), "bucket 0"))) {
+      cleanup_tmp();
+      return fail(stt);
+    }
+    if (c0 > 0) {
+      uint64_t real00 = 0;
+      if (n_refs && stop[0] > 0 && cut0 > 0) {
+        std::vector<uint8_t> w(k);
+        for (int i = 0; i < k; i++) w[i] = host_upper(h_ref_bytes[base + i]);
+        real00 = host_hash28(w.data(), k) == 0 ? 1 : 0;
+      }
+      ix->phantom = phantom + real00 - 1;
+      const uint32_t stored = c0 - (uint32_t)ix->phantom;
+      hipMemcpyAsync(d_cnt, &stored, sizeof(uint32_t), hipMemcpyHostToDevice, st);
+      if ((stt = check(hipStreamSynchronize(st), "bucket 0 count"))) {
+        cleanup_tmp();
+        return fail(stt);
+      }
+    }
+  }
   // Exclusive scan of counts into u64 offsets.
   {
     hipcub::TransformInputIterator<uint64_t, ToU64, const uint32_t *> in(d_cnt, ToU64());
@@ -307,7 +337,7 @@ mh_status build_index(mh_handle *h, const uint8_t *h_ref_bytes, const uint64_t *
     cleanup_tmp();
     return fail(stt);
   }
-  ix->n_entries = n_ent;
+  ix->n_entries = n_ent + ix->phantom;
   if ((stt = check(hipMalloc(&ix->d_ent, std::max<uint64_t>(n_ent, 1) * sizeof(uint2)), "ent"))) {
     cleanup_tmp();
     return fail(stt);

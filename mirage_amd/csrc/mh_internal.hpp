@@ -27,6 +27,13 @@ struct IndexView {
   uint32_t n_refs;
   uint32_t empty_hash;      // present bucket with no entries, or kNoHash
   int32_t T, s, k;
+  // Copies of the entry (ref 0, pos 0) that bucket 0 holds beyond the one
+  // stored in ent[]: the windows dropped after "N"-window cutoffs, which the
+  // reference keeps as identical entries of hash 0.  Identical pairs never
+  // chain, so each extra copy only adds the one-pair seed (0, j, s, j + s)
+  // for every probe j that hit bucket 0; the kernels weight that seed instead
+  // of storing the copies.
+  uint32_t phantom;
 };
 
 // Tables of glibc logf for the relative score (MatrixResult.cpp:21-29):
@@ -102,7 +109,8 @@ struct mh_index {
   int32_t threshold = 0;
   uint32_t n_refs = 0;
   uint32_t empty_hash = mh::kNoHash;
-  uint64_t n_entries = 0;
+  uint64_t n_entries = 0;   // entries of the reference's index (stored + phantom)
+  uint64_t phantom = 0;     // copies of (0, 0) in bucket 0 not stored in d_ent
   uint64_t n_buckets = 0;
   uint64_t total_bytes = 0;
   uint64_t *d_dir = nullptr;
@@ -121,6 +129,7 @@ struct mh_index {
     v.T = threshold;
     v.s = threshold / 2;
     v.k = threshold - threshold / 2 + 1;
+    v.phantom = (uint32_t)phantom;
     return v;
   }
 };

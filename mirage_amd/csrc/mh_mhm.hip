@@ -125,9 +125,9 @@ __device__ inline void flag_bad(unsigned long long *err, uint64_t i) {
 // Pass 1: entry counts of the records into the hash-indexed counts; records
 // with more than kSmallRecord entries are listed for the workgroup pass.
 __global__ void k_mhm_count(const uint8_t *rec, const uint64_t *sp, const uint32_t *hash,
-                            uint64_t n, uint64_t skip_last, uint32_t *cnt,
+                            uint64_t first, uint64_t n, uint64_t skip_last, uint32_t *cnt,
                             unsigned long long *err, uint32_t *big, unsigned int *n_big) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+  for (uint64_t i = first + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     if (skip_last && i == n - 1) continue;
     const RecHead hd = rec_head(rec + sp[i], sp[i + 1] - sp[i]);
@@ -142,10 +142,10 @@ __global__ void k_mhm_count(const uint8_t *rec, const uint64_t *sp, const uint32
 
 // Pass 2: a thread decodes a record of at most kSmallRecord entries.
 __global__ void k_mhm_decode_small(const uint8_t *rec, const uint64_t *sp, const uint32_t *hash,
-                                   uint64_t n, uint64_t skip_last, const uint64_t *dir,
-                                   const uint64_t *ref_off, uint32_t n_refs, uint2 *ent,
-                                   unsigned long long *err) {
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+                                   uint64_t first, uint64_t n, uint64_t skip_last,
+                                   const uint64_t *dir, const uint64_t *ref_off, uint32_t n_refs,
+                                   uint2 *ent, unsigned long long *err) {
+  for (uint64_t i = first + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
     if (skip_last && i == n - 1) continue;
     const uint8_t *r = rec + sp[i];
@@ -513,6 +513,36 @@ bool decode_host(const mh_mhm *m, uint64_t i, std::vector<uint2> &out) {
   return true;
 }
 
+// Host decode of the record of hash 0 (TOC entry 0) with its identical
+// (0, 0) entries counted instead of stored: out receives the other entries
+// and one (0, 0) when there is any; *copies the number of (0, 0) entries.
+bool reduce_bucket0(const mh_mhm *m, std::vector<uint2> &out, uint64_t *copies) {
+  const uint8_t *r = m->p + m->sp[0];
+  const RecHead hd = rec_head(r, m->sp[1] - m->sp[0]);
+  if (!hd.ok || hd.n > 0xFFFFFFFFull) return false;
+  const uint32_t n_refs = (uint32_t)m->seq_size.size();
+  out.clear();
+  *copies = 0;
+  uint64_t j = 0;
+  for (uint64_t t = 0; t < hd.nrun; t++) {
+    const uint8_t *q = r + hd.runs_at + t * (hd.brep + hd.bpos);
+    const uint64_t c = get_le(q, hd.brep);
+    const uint64_t pv = get_le(q + hd.brep, hd.bpos);
+    if (c > hd.n - j || pv > 0xFFFFFFFFull - hd.pos_min) return false;
+    const uint32_t pos = (uint32_t)(pv + hd.pos_min);
+    for (uint64_t u = 0; u < c; u++, j++) {
+      const uint64_t id = ref_id(get_le(r + hd.ids_at + j * hd.bid, hd.bid), hd.id_min, n_refs);
+      if (id == ~0ull || !pos_ok(pos, m->seq_size[id])) return false;
+      if (id == 0 && pos == 0) {
+        if ((*copies)++ == 0) out.push_back(make_uint2(0, 0));
+      } else {
+        out.push_back(make_uint2((uint32_t)id, pos));
+      }
+    }
+  }
+  return j == hd.n;
+}
+
 }  // namespace
 
 extern "C" {
@@ -700,10 +730,23 @@ mh_status mh_index_from_mhm(mh_handle *h, const mh_mhm *m, int32_t threshold, ui
   MHM_HIP(hipMemsetAsync(d_cnt, 0, (kDirSize + 1) * sizeof(uint32_t), st));
   const unsigned long long init[4] = {~0ull, 0, 0, 0};
   MHM_HIP(hipMemcpyAsync(d_misc, init, sizeof(init), hipMemcpyHostToDevice, st));
-  if (n)
-    k_mhm_count<<<grid_of(n, 256), 256, 0, st>>>(d_rec, d_sp, d_hash, n, skip_last ? 1 : 0,
-                                                 d_cnt, d_misc, d_big,
-                                                 (unsigned int *)(d_misc + 1));
+  // Bucket 0 is decoded here: its identical (0, 0) entries, one per window
+  // the reference dropped after an "N"-window cutoff, are stored once and
+  // counted in ix->phantom.
+  const uint64_t first = (n && m->hash[0] == 0 && !(skip_last && n == 1)) ? 1 : 0;
+  std::vector<uint2> b0;
+  if (first) {
+    uint64_t copies = 0;
+    if (!reduce_bucket0(m, b0, &copies))
+      return fail(set_error(h, MH_ERR_ARG, "mh_index_from_mhm: corrupt record 0 (hash 0)"));
+    ix->phantom = copies > 1 ? copies - 1 : 0;
+    const uint32_t c0 = (uint32_t)b0.size();
+    MHM_HIP(hipMemcpyAsync(d_cnt, &c0, sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  }
+  if (n > first)
+    k_mhm_count<<<grid_of(n - first, 256), 256, 0, st>>>(d_rec, d_sp, d_hash, first, n,
+                                                         skip_last ? 1 : 0, d_cnt, d_misc, d_big,
+                                                         (unsigned int *)(d_misc + 1));
   {
     hipcub::TransformInputIterator<uint64_t, ToU64, const uint32_t *> in(d_cnt, ToU64());
     size_t tmp_bytes = 0;
@@ -723,13 +766,15 @@ mh_status mh_index_from_mhm(mh_handle *h, const mh_mhm *m, int32_t threshold, ui
                               " (hash " + std::to_string(m->hash[i]) + ")"));
   };
   if (misc[0] != ~0ull) return corrupt(misc[0]);
-  ix->n_entries = n_ent;
+  ix->n_entries = n_ent + ix->phantom;
   MHM_HIP(hipMalloc(&ix->d_ent, std::max<uint64_t>(n_ent, 1) * sizeof(uint2)));
-  if (n) {
-    k_mhm_decode_small<<<grid_of(n, 256), 256, 0, st>>>(d_rec, d_sp, d_hash, n,
-                                                        skip_last ? 1 : 0, ix->d_dir,
-                                                        ix->d_ref_off, n_refs, ix->d_ent,
-                                                        d_misc);
+  if (!b0.empty())  // bucket 0 starts at entry 0
+    MHM_HIP(hipMemcpyAsync(ix->d_ent, b0.data(), b0.size() * sizeof(uint2),
+                           hipMemcpyHostToDevice, st));
+  if (n > first) {
+    k_mhm_decode_small<<<grid_of(n - first, 256), 256, 0, st>>>(
+        d_rec, d_sp, d_hash, first, n, skip_last ? 1 : 0, ix->d_dir, ix->d_ref_off, n_refs,
+        ix->d_ent, d_misc);
     const unsigned n_big = (unsigned)misc[1];
     if (n_big)
       k_mhm_decode_big<<<std::min(n_big, 4096u), kBigThreads, 0, st>>>(
@@ -771,9 +816,10 @@ mh_status mh_index_save_mhm(mh_handle *h, const mh_index *idx, const char *path,
   const uint64_t total = idx->h_ref_off[n_refs];
 
   // Non-empty buckets in ascending hash order with their entry offsets.
+  const uint64_t n_stored = idx->n_entries - idx->phantom;
   std::vector<uint32_t> hashes(nb);
   std::vector<uint64_t> begin(nb + 1);
-  std::vector<uint2> ent(idx->n_entries);
+  std::vector<uint2> ent(n_stored);
   std::vector<uint8_t> bytes(total);
   {
     uint32_t *d_sel = nullptr;
@@ -813,8 +859,8 @@ mh_status mh_index_save_mhm(mh_handle *h, const mh_index *idx, const char *path,
         e = hipMemcpyAsync(begin.data(), d_begin, nb * sizeof(uint64_t), hipMemcpyDeviceToHost,
                            st);
     }
-    if (e == hipSuccess && idx->n_entries)
-      e = hipMemcpyAsync(ent.data(), idx->d_ent, idx->n_entries * sizeof(uint2),
+    if (e == hipSuccess && n_stored)
+      e = hipMemcpyAsync(ent.data(), idx->d_ent, n_stored * sizeof(uint2),
                          hipMemcpyDeviceToHost, st);
     if (e == hipSuccess && total)
       e = hipMemcpyAsync(bytes.data(), idx->d_bytes, total, hipMemcpyDeviceToHost, st);
@@ -822,16 +868,30 @@ mh_status mh_index_save_mhm(mh_handle *h, const mh_index *idx, const char *path,
     release();
     if (e != hipSuccess) return hip_error(h, e, "mh_index_save_mhm: copy from device");
   }
-  begin[nb] = idx->n_entries;
+  begin[nb] = n_stored;
+  // Bucket 0 with its identical (0, 0) copies written out, as the reference
+  // holds them.
+  std::vector<uint2> b0;
+  const bool expand0 = nb && hashes[0] == 0 && idx->phantom;
+  if (expand0) {
+    b0.assign(ent.begin() + begin[0], ent.begin() + begin[1]);
+    b0.resize(b0.size() + idx->phantom, make_uint2(0, 0));
+  }
+  auto bucket = [&](uint64_t b) -> uint2 * {
+    return (b == 0 && expand0) ? b0.data() : ent.data() + begin[b];
+  };
+  auto bucket_n = [&](uint64_t b) -> uint64_t {
+    return (b == 0 && expand0) ? b0.size() : begin[b + 1] - begin[b];
+  };
 
   // Records: each bucket sorted by position (SortAndCorrectPairVector,
   // Hashmap.cpp:1561-1573; ties by reference id), sized, then written.
   std::vector<RecPlan> plan(nb);
   uint64_t max_bucket = 0;
-  for (uint64_t b = 0; b < nb; b++) max_bucket = std::max(max_bucket, begin[b + 1] - begin[b]);
+  for (uint64_t b = 0; b < nb; b++) max_bucket = std::max(max_bucket, bucket_n(b));
   parallel_for(nb, [&](uint64_t b) {
-    uint2 *e = ent.data() + begin[b];
-    const uint64_t n = begin[b + 1] - begin[b];
+    uint2 *e = bucket(b);
+    const uint64_t n = bucket_n(b);
     std::sort(e, e + n, [](const uint2 &a, const uint2 &c) {
       return a.y < c.y || (a.y == c.y && a.x < c.x);
     });
@@ -902,10 +962,7 @@ mh_status mh_index_save_mhm(mh_handle *h, const mh_index *idx, const char *path,
   std::vector<uint64_t> rec_at(nb + 1, 0);
   for (uint64_t b = 0; b < nb; b++) rec_at[b + 1] = rec_at[b] + plan[b].bytes;
   std::vector<uint8_t> recs(rec_at[nb]);
-  parallel_for(nb, [&](uint64_t b) {
-    write_record(recs.data() + rec_at[b], ent.data() + begin[b], begin[b + 1] - begin[b],
-                 plan[b]);
-  });
+  parallel_for(nb, [&](uint64_t b) { write_record(recs.data() + rec_at[b], bucket(b), bucket_n(b), plan[b]); });
 
   FILE *f = std::fopen(path, "wb");
   if (!f) return set_error(h, MH_ERR_ARG, std::string("mh_index_save_mhm: cannot create ") + path);
