@@ -263,6 +263,22 @@ __global__ __launch_bounds__(kBigThreads) void k_mhm_decode_big(
   }
 }
 
+// TOC entries (hash [hb], record position [sb]) into aligned columns;
+// positions become offsets into the record bytes, sp[n] their total.
+__global__ void k_mhm_toc(const uint8_t *toc, uint64_t n, uint32_t hb, uint32_t sb,
+                          uint64_t rec0, uint64_t rec_bytes, uint32_t *hash, uint64_t *sp) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i <= n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    if (i == n) {
+      sp[n] = rec_bytes;
+      continue;
+    }
+    const uint8_t *e = toc + i * (hb + sb);
+    hash[i] = (uint32_t)get_le(e, hb);
+    sp[i] = get_le(e + hb, sb) - rec0;
+  }
+}
+
 __global__ void k_upper_bytes(uint8_t *b, uint64_t n) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * blockDim.x) {
@@ -313,10 +329,11 @@ uint8_t *put_le(uint8_t *p, uint64_t v, uint32_t w) {
   return p;
 }
 
+// f(i) for i < n on up to 16 host threads, which claim `grain` items at a time.
 template <class F>
-void parallel_for(uint64_t n, F &&f) {
+void parallel_for(uint64_t n, F &&f, uint64_t grain = 1024) {
   const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  if (n < 4096 || nt == 1) {
+  if (n <= grain || nt == 1) {
     for (uint64_t i = 0; i < n; i++) f(i);
     return;
   }
@@ -325,9 +342,9 @@ void parallel_for(uint64_t n, F &&f) {
   for (unsigned t = 0; t < nt; t++)
     th.emplace_back([&]() {
       for (;;) {
-        const uint64_t b = next.fetch_add(1024);
+        const uint64_t b = next.fetch_add(grain);
         if (b >= n) return;
-        const uint64_t e = std::min<uint64_t>(n, b + 1024);
+        const uint64_t e = std::min<uint64_t>(n, b + grain);
         for (uint64_t i = b; i < e; i++) f(i);
       }
     });
@@ -394,9 +411,14 @@ struct mh_mhm {
   uint64_t size = 0;
   int32_t threshold = 0;
   uint32_t hb = 4, sb = 4;
-  uint64_t n = 0;               // TOC entries
-  std::vector<uint32_t> hash;   // ascending
-  std::vector<uint64_t> sp;     // n + 1 record positions, the last = file size
+  uint64_t n = 0;        // TOC entries
+  uint64_t toc_off = 0;  // file position of the first TOC entry
+  // TOC entry i, read from the mapped file (validated at open: ascending
+  // hashes below 2^28, ascending positions inside the file).
+  uint64_t hash(uint64_t i) const { return get_le(p + toc_off + i * (hb + sb), hb); }
+  uint64_t sp(uint64_t i) const {  // sp(n) is the file size
+    return i == n ? size : get_le(p + toc_off + i * (hb + sb) + hb, sb);
+  }
   std::vector<int32_t> taxids;
   std::vector<uint64_t> seq_size, seq_sp, hdr_size, hdr_sp;
   uint64_t ref_bytes = 0, hdr_bytes = 0;
@@ -444,17 +466,8 @@ mh_status parse(mh_mhm *m) {
   if (m->threshold < 2) return bad_file("threshold " + std::to_string(m->threshold));
   m->hb = (uint32_t)hb;
   if (m->n > (m->size - c.o) / (m->hb + m->sb)) return bad_file("table of contents past the end");
-  m->hash.resize(m->n);
-  m->sp.resize(m->n + 1);
-  for (uint64_t i = 0; i < m->n; i++) {
-    const uint64_t h = c.get(m->hb);
-    m->sp[i] = c.get(m->sb);
-    if (h > kHashMask) return bad_file("hash above 28 bits in the table of contents");
-    if (i && h <= m->hash[i - 1]) return bad_file("table of contents not in ascending hash order");
-    if (i && m->sp[i] <= m->sp[i - 1]) return bad_file("record positions not ascending");
-    m->hash[i] = (uint32_t)h;
-  }
-  m->sp[m->n] = m->size;
+  m->toc_off = c.o;
+  c.o += m->n * (m->hb + m->sb);
   const uint64_t ntax = c.get(m->hb);
   if (!c.ok || ntax > (m->size - c.o) / 4) return bad_file("taxonomy ids past the end");
   m->taxids.resize(ntax);
@@ -481,15 +494,38 @@ mh_status parse(mh_mhm *m) {
     (part ? m->hdr_bytes : m->ref_bytes) = total;
   }
   if (!c.ok) return bad_file("truncated file");
-  if (m->n && m->sp[0] < c.o) return bad_file("record positions inside the tables");
-  if (m->n && m->sp[m->n - 1] >= m->size) return bad_file("record positions past the end");
+  if (m->n && m->sp(0) < c.o) return bad_file("record positions inside the tables");
+  if (m->n && m->sp(m->n - 1) >= m->size) return bad_file("record positions past the end");
+  // The TOC, in parallel slices: hashes below 2^28 and both columns ascending.
+  std::atomic<int> bad{0};  // 1 hash range, 2 hash order, 3 position order
+  parallel_for((m->n + 65535) / 65536, [&](uint64_t b) {
+    const uint64_t i0 = b * 65536, i1 = std::min<uint64_t>(m->n, i0 + 65536);
+    uint64_t ph = i0 ? m->hash(i0 - 1) : 0, ps = i0 ? m->sp(i0 - 1) : 0;
+    for (uint64_t i = i0; i < i1; i++) {
+      const uint64_t h = m->hash(i), sp = m->sp(i);
+      int e = 0;
+      if (h > kHashMask) e = 1;
+      else if (i && h <= ph) e = 2;
+      else if (i && sp <= ps) e = 3;
+      if (e) {
+        int z = 0;
+        bad.compare_exchange_strong(z, e);
+        return;
+      }
+      ph = h;
+      ps = sp;
+    }
+  }, 1);
+  if (bad == 1) return bad_file("hash above 28 bits in the table of contents");
+  if (bad == 2) return bad_file("table of contents not in ascending hash order");
+  if (bad == 3) return bad_file("record positions not ascending");
   return MH_OK;
 }
 
 // Host decode of TOC entry i into (id, pos) pairs; false on a bad record.
 bool decode_host(const mh_mhm *m, uint64_t i, std::vector<uint2> &out) {
-  const uint8_t *r = m->p + m->sp[i];
-  const RecHead hd = rec_head(r, m->sp[i + 1] - m->sp[i]);
+  const uint8_t *r = m->p + m->sp(i);
+  const RecHead hd = rec_head(r, m->sp(i + 1) - m->sp(i));
   if (!hd.ok || hd.n > 0xFFFFFFFFull) return false;
   const uint32_t n_refs = (uint32_t)m->seq_size.size();
   out.assign(hd.n, make_uint2(0, 0));
@@ -517,8 +553,8 @@ bool decode_host(const mh_mhm *m, uint64_t i, std::vector<uint2> &out) {
 // (0, 0) entries counted instead of stored: out receives the other entries
 // and one (0, 0) when there is any; *copies the number of (0, 0) entries.
 bool reduce_bucket0(const mh_mhm *m, std::vector<uint2> &out, uint64_t *copies) {
-  const uint8_t *r = m->p + m->sp[0];
-  const RecHead hd = rec_head(r, m->sp[1] - m->sp[0]);
+  const uint8_t *r = m->p + m->sp(0);
+  const RecHead hd = rec_head(r, m->sp(1) - m->sp(0));
   if (!hd.ok || hd.n > 0xFFFFFFFFull) return false;
   const uint32_t n_refs = (uint32_t)m->seq_size.size();
   out.clear();
@@ -541,6 +577,69 @@ bool reduce_bucket0(const mh_mhm *m, std::vector<uint2> &out, uint64_t *copies) 
     }
   }
   return j == hd.n;
+}
+
+// Copies file bytes [off, off + len) to device memory through pinned
+// buffers: host threads read 16 MiB chunks with pread, each issuing its
+// copies on a stream of its own, with two buffers per thread so that reading
+// the next chunk overlaps the transfer of the last.  Returns 0, -1 when the
+// file could not be read, or the first HIP error.
+int upload_span(const mh_mhm *m, int device, uint64_t off, uint64_t len, uint8_t *d_dst) {
+  if (len == 0) return 0;
+  constexpr uint64_t kChunk = 16ull << 20;
+  const uint64_t nchunk = (len + kChunk - 1) / kChunk;
+  const unsigned hw = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+  const unsigned nt = (unsigned)std::min<uint64_t>(nchunk, hw);
+  std::atomic<int> err{0};
+  auto note = [&](int e) {
+    int z = 0;
+    err.compare_exchange_strong(z, e);
+  };
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nt; t++)
+    th.emplace_back([&, t]() {
+      hipSetDevice(device);
+      hipStream_t s = nullptr;
+      uint8_t *buf[2] = {nullptr, nullptr};
+      hipEvent_t ev[2] = {nullptr, nullptr};
+      bool used[2] = {false, false};
+      hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+      for (int b = 0; b < 2 && e == hipSuccess; b++) {
+        e = hipHostMalloc((void **)&buf[b], kChunk, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[b], hipEventDisableTiming);
+      }
+      int par = 0;
+      for (uint64_t c = t; c < nchunk && e == hipSuccess && err.load() == 0; c += nt) {
+        if (used[par] && (e = hipEventSynchronize(ev[par])) != hipSuccess) break;
+        const uint64_t o = c * kChunk, want = std::min(kChunk, len - o);
+        uint64_t got = 0;
+        while (got < want) {
+          const ssize_t r = pread(m->fd, buf[par] + got, want - got, (off_t)(off + o + got));
+          if (r <= 0) break;
+          got += (uint64_t)r;
+        }
+        if (got < want) {
+          note(-1);
+          break;
+        }
+        e = hipMemcpyAsync(d_dst + o, buf[par], want, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipEventRecord(ev[par], s);
+        used[par] = true;
+        par ^= 1;
+      }
+      if (s) {
+        const hipError_t e2 = hipStreamSynchronize(s);
+        if (e == hipSuccess) e = e2;
+      }
+      for (int b = 0; b < 2; b++) {
+        if (ev[b]) hipEventDestroy(ev[b]);
+        if (buf[b]) hipHostFree(buf[b]);
+      }
+      if (s) hipStreamDestroy(s);
+      if (e != hipSuccess) note((int)e);
+    });
+  for (auto &x : th) x.join();
+  return err.load();
 }
 
 }  // namespace
@@ -584,7 +683,7 @@ mh_status mh_mhm_get_info(const mh_mhm *m, mh_mhm_info *out) {
   if (!m || !out) return set_error(nullptr, MH_ERR_ARG, "mh_mhm_get_info: bad argument");
   out->n_buckets = m->n;
   out->file_bytes = m->size;
-  out->record_bytes = m->n ? m->size - m->sp[0] : 0;
+  out->record_bytes = m->n ? m->size - m->sp(0) : 0;
   out->ref_bytes = m->ref_bytes;
   out->header_bytes = m->hdr_bytes;
   out->n_refs = (uint32_t)m->seq_size.size();
@@ -618,12 +717,17 @@ mh_status mh_mhm_references(const mh_mhm *m, uint8_t *h_bytes, uint64_t *h_off,
 mh_status mh_mhm_bucket(const mh_mhm *m, uint32_t hash, uint32_t flags, uint32_t *h_refs,
                         uint32_t *h_pos, uint64_t cap, int64_t *count) {
   if (!m || !count) return set_error(nullptr, MH_ERR_ARG, "mh_mhm_bucket: bad argument");
-  auto it = std::lower_bound(m->hash.begin(), m->hash.end(), hash);
-  if (it == m->hash.end() || *it != hash) {
+  uint64_t lo = 0, hi = m->n;  // first entry with hash >= the key
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) / 2;
+    if (m->hash(mid) < hash) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo == m->n || m->hash(lo) != hash) {
     *count = -1;
     return MH_OK;
   }
-  const uint64_t i = (uint64_t)(it - m->hash.begin());
+  const uint64_t i = lo;
   if ((flags & MH_INDEX_LAST_BUCKET_EMPTY) && i == m->n - 1) {
     *count = 0;
     return MH_OK;
@@ -669,13 +773,14 @@ mh_status mh_index_from_mhm(mh_handle *h, const mh_mhm *m, int32_t threshold, ui
     if (m->seq_sp[r] != m->seq_sp[0] + ix->h_ref_off[r]) contiguous = false;
   }
 
-  uint8_t *d_rec = nullptr;
+  uint8_t *d_rec = nullptr, *d_toc = nullptr;
   uint64_t *d_sp = nullptr;
   uint32_t *d_hash = nullptr, *d_cnt = nullptr, *d_big = nullptr;
   unsigned long long *d_misc = nullptr;  // [0] first bad record, [1] big-record count
   void *d_tmp = nullptr;
   auto cleanup = [&]() {
     if (d_rec) hipFree(d_rec);
+    if (d_toc) hipFree(d_toc);
     if (d_sp) hipFree(d_sp);
     if (d_hash) hipFree(d_hash);
     if (d_cnt) hipFree(d_cnt);
@@ -695,45 +800,47 @@ mh_status mh_index_from_mhm(mh_handle *h, const mh_mhm *m, int32_t threshold, ui
     if (e_ != hipSuccess) return fail(hip_error(h, e_, #call)); \
   } while (0)
 
+  const uint64_t rec0 = n ? m->sp(0) : m->size;
+  const uint64_t rec_bytes = m->size - rec0;
+  const uint64_t toc_bytes = n * (m->hb + m->sb);
   MHM_HIP(hipMalloc(&ix->d_bytes, std::max<uint64_t>(total, 1) + 64));
   MHM_HIP(hipMalloc(&ix->d_ref_off, (n_refs + 1) * sizeof(uint64_t)));
   MHM_HIP(hipMalloc(&ix->d_dir, (kDirSize + 1) * sizeof(uint64_t)));
-  MHM_HIP(hipMemcpyAsync(ix->d_ref_off, ix->h_ref_off.data(), (n_refs + 1) * sizeof(uint64_t),
-                         hipMemcpyHostToDevice, st));
-  if (total) {
-    if (contiguous) {
-      MHM_HIP(hipMemcpyAsync(ix->d_bytes, m->p + m->seq_sp[0], total, hipMemcpyHostToDevice, st));
-    } else {
-      for (uint32_t r = 0; r < n_refs; r++)
-        if (m->seq_size[r])
-          MHM_HIP(hipMemcpyAsync(ix->d_bytes + ix->h_ref_off[r], m->p + m->seq_sp[r],
-                                 m->seq_size[r], hipMemcpyHostToDevice, st));
-    }
-    k_upper_bytes<<<grid_of(total, 256), 256, 0, st>>>(ix->d_bytes, total);
-  }
-  const uint64_t rec_bytes = n ? m->size - m->sp[0] : 0;
-  std::vector<uint64_t> sp_rel(n + 1);
-  for (uint64_t i = 0; i <= n; i++) sp_rel[i] = n ? m->sp[i] - m->sp[0] : 0;
   MHM_HIP(hipMalloc(&d_rec, std::max<uint64_t>(rec_bytes, 1)));
+  MHM_HIP(hipMalloc(&d_toc, std::max<uint64_t>(toc_bytes, 1)));
   MHM_HIP(hipMalloc(&d_sp, (n + 1) * sizeof(uint64_t)));
   MHM_HIP(hipMalloc(&d_hash, std::max<uint64_t>(n, 1) * sizeof(uint32_t)));
   MHM_HIP(hipMalloc(&d_big, std::max<uint64_t>(n, 1) * sizeof(uint32_t)));
   MHM_HIP(hipMalloc(&d_cnt, (kDirSize + 1) * sizeof(uint32_t)));
   MHM_HIP(hipMalloc(&d_misc, 4 * sizeof(unsigned long long)));
-  if (rec_bytes)
-    MHM_HIP(hipMemcpyAsync(d_rec, m->p + m->sp[0], rec_bytes, hipMemcpyHostToDevice, st));
-  MHM_HIP(hipMemcpyAsync(d_sp, sp_rel.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
-                         st));
-  if (n)
-    MHM_HIP(hipMemcpyAsync(d_hash, m->hash.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice,
-                           st));
+  MHM_HIP(hipMemcpyAsync(ix->d_ref_off, ix->h_ref_off.data(), (n_refs + 1) * sizeof(uint64_t),
+                         hipMemcpyHostToDevice, st));
+  // The bulk of the file goes up as stored: sequences, TOC, records.
+  {
+    int e = 0;
+    if (total && contiguous) {
+      e = upload_span(m, h->device, m->seq_sp[0], total, ix->d_bytes);
+    } else {
+      for (uint32_t r = 0; r < n_refs && e == 0; r++)
+        if (m->seq_size[r])
+          e = (int)hipMemcpy(ix->d_bytes + ix->h_ref_off[r], m->p + m->seq_sp[r],
+                             m->seq_size[r], hipMemcpyHostToDevice);
+    }
+    if (e == 0) e = upload_span(m, h->device, m->toc_off, toc_bytes, d_toc);
+    if (e == 0) e = upload_span(m, h->device, rec0, rec_bytes, d_rec);
+    if (e == -1) return fail(set_error(h, MH_ERR_ARG, "mh_index_from_mhm: file read failed"));
+    if (e) return fail(hip_error(h, (hipError_t)e, "mh_index_from_mhm: upload"));
+  }
+  if (total) k_upper_bytes<<<grid_of(total, 256), 256, 0, st>>>(ix->d_bytes, total);
+  k_mhm_toc<<<grid_of(n + 1, 256), 256, 0, st>>>(d_toc, n, m->hb, m->sb, rec0, rec_bytes, d_hash,
+                                                  d_sp);
   MHM_HIP(hipMemsetAsync(d_cnt, 0, (kDirSize + 1) * sizeof(uint32_t), st));
   const unsigned long long init[4] = {~0ull, 0, 0, 0};
   MHM_HIP(hipMemcpyAsync(d_misc, init, sizeof(init), hipMemcpyHostToDevice, st));
   // Bucket 0 is decoded here: its identical (0, 0) entries, one per window
   // the reference dropped after an "N"-window cutoff, are stored once and
   // counted in ix->phantom.
-  const uint64_t first = (n && m->hash[0] == 0 && !(skip_last && n == 1)) ? 1 : 0;
+  const uint64_t first = (n && m->hash(0) == 0 && !(skip_last && n == 1)) ? 1 : 0;
   std::vector<uint2> b0;
   if (first) {
     uint64_t copies = 0;
@@ -763,7 +870,7 @@ mh_status mh_index_from_mhm(mh_handle *h, const mh_mhm *m, int32_t threshold, ui
   auto corrupt = [&](unsigned long long i) {
     return fail(set_error(h, MH_ERR_ARG,
                           "mh_index_from_mhm: corrupt record " + std::to_string(i) +
-                              " (hash " + std::to_string(m->hash[i]) + ")"));
+                              " (hash " + std::to_string(m->hash(i)) + ")"));
   };
   if (misc[0] != ~0ull) return corrupt(misc[0]);
   ix->n_entries = n_ent + ix->phantom;
@@ -787,7 +894,7 @@ mh_status mh_index_from_mhm(mh_handle *h, const mh_mhm *m, int32_t threshold, ui
 #undef MHM_HIP
   cleanup();
   ix->n_buckets = skip_last ? n - 1 : n;
-  if (skip_last) ix->empty_hash = m->hash[n - 1];
+  if (skip_last) ix->empty_hash = (uint32_t)m->hash(n - 1);
   ix->total_bytes = (std::max<uint64_t>(total, 1) + 64) + (n_refs + 1) * 8 +
                     (kDirSize + 1) * 8 + std::max<uint64_t>(n_ent, 1) * 8;
   *out = ix;
