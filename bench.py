@@ -169,7 +169,10 @@ def mhm_leg(torch, native, handle, ix, info, rb, ro, threshold, d_reads, d_off, 
     finally:
         full.free()
     size = os.path.getsize(path)
-    log(f"mhm: wrote {size / 2**30:.2f} GiB in {save_s:.2f} s")
+    with native.MhmFile(path) as f:
+        finfo = f.info()
+    log(f"mhm: wrote {size / 2**30:.2f} GiB in {save_s:.2f} s ({finfo['n_buckets']} buckets, "
+        f"{finfo['record_bytes'] / 2**30:.2f} GiB of records)")
     loads, same_index, same_results = [], True, None
     keys = ("n_entries", "n_buckets", "empty_hash", "n_refs", "threshold")
     try:
@@ -198,7 +201,9 @@ def mhm_leg(torch, native, handle, ix, info, rb, ro, threshold, d_reads, d_off, 
     best = min(loads)
     log(f"mhm: load {best:.3f} s ({size / best / 1e9:.2f} GB/s of file), index same "
         f"{same_index}, results same {same_results}")
-    return {"file_bytes": size, "save_s": round(save_s, 3), "load_s": round(best, 4),
+    return {"file_bytes": size, "toc_entries": finfo["n_buckets"],
+            "record_bytes": finfo["record_bytes"], "save_s": round(save_s, 3),
+            "load_s": round(best, 4),
             "load_s_all": [round(x, 4) for x in loads],
             "load_file_GBps": round(size / best / 1e9, 3), "dir": d,
             "index_same_as_build": bool(same_index), "results_same": same_results}

@@ -36,6 +36,7 @@
 #include <atomic>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <thread>
 #include <vector>
@@ -320,10 +321,6 @@ uint32_t nbytes(uint64_t v) {
   return bits == 0 ? 1 : (bits - 1) / 8 + 1;
 }
 
-void put_le(std::vector<uint8_t> &out, uint64_t v, uint32_t w) {
-  for (uint32_t i = 0; i < w; i++) out.push_back((uint8_t)(v >> (8 * i)));
-}
-
 uint8_t *put_le(uint8_t *p, uint64_t v, uint32_t w) {
   for (uint32_t i = 0; i < w; i++) *p++ = (uint8_t)(v >> (8 * i));
   return p;
@@ -350,6 +347,19 @@ void parallel_for(uint64_t n, F &&f, uint64_t grain = 1024) {
     });
   for (auto &t : th) t.join();
 }
+
+// Host array left uninitialised (every byte is written before it is read):
+// the writer's buffers run to gigabytes, and zeroing them first costs seconds.
+template <class T>
+struct HostBuf {
+  std::unique_ptr<T[]> p;
+  uint64_t n = 0;
+  explicit HostBuf(uint64_t count) : p(new T[count ? count : 1]), n(count) {}
+  T *data() { return p.get(); }
+  const T *data() const { return p.get(); }
+  uint64_t size() const { return n; }
+  T *begin() { return p.get(); }
+};
 
 // Size of one bucket's record for entries sorted by position
 // (GetSaveHashmapPreCalculates, Hashmap.cpp:1620-1697, written at :2435-2516).
@@ -926,8 +936,8 @@ mh_status mh_index_save_mhm(mh_handle *h, const mh_index *idx, const char *path,
   const uint64_t n_stored = idx->n_entries - idx->phantom;
   std::vector<uint32_t> hashes(nb);
   std::vector<uint64_t> begin(nb + 1);
-  std::vector<uint2> ent(n_stored);
-  std::vector<uint8_t> bytes(total);
+  HostBuf<uint2> ent(n_stored);
+  HostBuf<uint8_t> bytes(total);
   {
     uint32_t *d_sel = nullptr;
     uint64_t *d_begin = nullptr;
@@ -999,9 +1009,10 @@ mh_status mh_index_save_mhm(mh_handle *h, const mh_index *idx, const char *path,
   parallel_for(nb, [&](uint64_t b) {
     uint2 *e = bucket(b);
     const uint64_t n = bucket_n(b);
-    std::sort(e, e + n, [](const uint2 &a, const uint2 &c) {
-      return a.y < c.y || (a.y == c.y && a.x < c.x);
-    });
+    if (n > 1)
+      std::sort(e, e + n, [](const uint2 &a, const uint2 &c) {
+        return a.y < c.y || (a.y == c.y && a.x < c.x);
+      });
     plan[b] = plan_record(e, n);
   });
 
@@ -1025,61 +1036,90 @@ mh_status mh_index_save_mhm(mh_handle *h, const mh_index *idx, const char *path,
     max_hdr = std::max(max_hdr, hdr_off[r + 1] - hdr_off[r]);
   }
 
-  // Everything before the records (Hashmap.cpp:1870-2240).
-  std::vector<uint8_t> head;
-  head.insert(head.end(), kIndicator, kIndicator + kIndicatorLen);
-  put_le(head, nb, 8);
-  put_le(head, (uint32_t)kVersion, 4);
-  put_le(head, hb, 4);
-  put_le(head, (uint32_t)idx->threshold, 4);
-  head.push_back((uint8_t)sb);
-  head.push_back((uint8_t)nbytes(max_bucket));
-  head.push_back((uint8_t)nbytes(max_seq));
-  head.push_back((uint8_t)nbytes(max_hdr));
+  // Record positions (Hashmap.cpp:1886-1920, :2064-2091).
+  std::vector<uint64_t> rec_at(nb + 1, 0);
+  for (uint64_t b = 0; b < nb; b++) rec_at[b + 1] = rec_at[b] + plan[b].bytes;
+
+  // Everything before the records (Hashmap.cpp:1870-2240): the header, the
+  // TOC (filled in parallel at fixed offsets), the taxonomy ids and the two
+  // reference tables.
+  const uint64_t head0 = kIndicatorLen + 8 + 12 + 4;
   const uint64_t meta = hb + 4ull * n_refs + 2 * (sb + hb + (uint64_t)n_refs * (hb + sb));
-  const uint64_t data0 = head.size() + nb * (hb + sb) + meta;
+  const uint64_t data0 = head0 + nb * (hb + sb) + meta;
   const uint64_t rec0 = data0 + total + hdr.size();
-  uint64_t pos = rec0;
-  const uint64_t sp_max = sb == 8 ? ~0ull : 0xFFFFFFFFull;
-  for (uint64_t b = 0; b < nb; b++) {
-    if (pos > sp_max)
-      return set_error(h, MH_ERR_RANGE,
-                       "mh_index_save_mhm: file too large for 4-byte stream positions");
-    put_le(head, hashes[b], hb);
-    put_le(head, pos, sb);
-    pos += plan[b].bytes;
-  }
-  const uint64_t file_bytes = pos;
-  put_le(head, n_refs, hb);
-  for (uint32_t r = 0; r < n_refs; r++) put_le(head, (uint32_t)(h_taxids ? h_taxids[r] : -1), 4);
+  if (sb == 4 && nb && rec0 + rec_at[nb - 1] > 0xFFFFFFFFull)
+    return set_error(h, MH_ERR_RANGE,
+                     "mh_index_save_mhm: file too large for 4-byte stream positions");
+  HostBuf<uint8_t> head(data0);
+  uint8_t *o = head.data();
+  std::memcpy(o, kIndicator, kIndicatorLen);
+  o += kIndicatorLen;
+  o = put_le(o, nb, 8);
+  o = put_le(o, (uint32_t)kVersion, 4);
+  o = put_le(o, hb, 4);
+  o = put_le(o, (uint32_t)idx->threshold, 4);
+  *o++ = (uint8_t)sb;
+  *o++ = (uint8_t)nbytes(max_bucket);
+  *o++ = (uint8_t)nbytes(max_seq);
+  *o++ = (uint8_t)nbytes(max_hdr);
+  parallel_for(nb, [&](uint64_t b) {
+    uint8_t *e = head.data() + head0 + b * (hb + sb);
+    e = put_le(e, hashes[b], hb);
+    put_le(e, rec0 + rec_at[b], sb);
+  }, 65536);
+  o = head.data() + head0 + nb * (hb + sb);
+  o = put_le(o, n_refs, hb);
+  for (uint32_t r = 0; r < n_refs; r++) o = put_le(o, (uint32_t)(h_taxids ? h_taxids[r] : -1), 4);
   uint64_t dpos = data0;
   for (int part = 0; part < 2; part++) {
     const uint64_t *off = part ? hdr_off.data() : idx->h_ref_off.data();
-    put_le(head, head.size() + sb + hb + (uint64_t)n_refs * (hb + sb), sb);
-    put_le(head, n_refs, hb);
-    for (uint32_t r = 0; r < n_refs; r++) put_le(head, off[r + 1] - off[r], hb);
+    o = put_le(o, (uint64_t)(o - head.data()) + sb + hb + (uint64_t)n_refs * (hb + sb), sb);
+    o = put_le(o, n_refs, hb);
+    for (uint32_t r = 0; r < n_refs; r++) o = put_le(o, off[r + 1] - off[r], hb);
     for (uint32_t r = 0; r < n_refs; r++) {
-      put_le(head, dpos, sb);
+      o = put_le(o, dpos, sb);
       dpos += off[r + 1] - off[r];
     }
   }
-  if (head.size() != data0 || dpos != rec0)
+  if ((uint64_t)(o - head.data()) != data0 || dpos != rec0)
     return set_error(h, MH_ERR_STATE, "mh_index_save_mhm: layout mismatch");
 
-  std::vector<uint64_t> rec_at(nb + 1, 0);
-  for (uint64_t b = 0; b < nb; b++) rec_at[b + 1] = rec_at[b] + plan[b].bytes;
-  std::vector<uint8_t> recs(rec_at[nb]);
-  parallel_for(nb, [&](uint64_t b) { write_record(recs.data() + rec_at[b], bucket(b), bucket_n(b), plan[b]); });
+  HostBuf<uint8_t> recs(rec_at[nb]);
+  parallel_for(nb, [&](uint64_t b) {
+    write_record(recs.data() + rec_at[b], bucket(b), bucket_n(b), plan[b]);
+  });
 
-  FILE *f = std::fopen(path, "wb");
-  if (!f) return set_error(h, MH_ERR_ARG, std::string("mh_index_save_mhm: cannot create ") + path);
-  bool ok = std::fwrite(head.data(), 1, head.size(), f) == head.size();
-  ok = ok && std::fwrite(bytes.data(), 1, bytes.size(), f) == bytes.size();
-  ok = ok && std::fwrite(hdr.data(), 1, hdr.size(), f) == hdr.size();
-  ok = ok && std::fwrite(recs.data(), 1, recs.size(), f) == recs.size();
-  ok = (std::fclose(f) == 0) && ok;
-  if (!ok || rec0 + recs.size() != file_bytes)
-    return set_error(h, MH_ERR_ARG, std::string("mh_index_save_mhm: write failed: ") + path);
+  // Four segments written at their offsets by parallel pwrite calls.
+  const int fd = open(path, O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0644);
+  if (fd < 0) return set_error(h, MH_ERR_ARG, std::string("mh_index_save_mhm: cannot create ") + path);
+  struct Seg {
+    const uint8_t *p;
+    uint64_t n, at;
+  };
+  const Seg segs[4] = {{head.data(), head.size(), 0},
+                       {bytes.data(), bytes.size(), data0},
+                       {hdr.data(), hdr.size(), data0 + total},
+                       {recs.data(), recs.size(), rec0}};
+  constexpr uint64_t kPiece = 64ull << 20;
+  std::vector<Seg> pieces;
+  for (const Seg &g : segs)
+    for (uint64_t x = 0; x < g.n; x += kPiece)
+      pieces.push_back({g.p + x, std::min(kPiece, g.n - x), g.at + x});
+  std::atomic<int> werr{0};
+  parallel_for(pieces.size(), [&](uint64_t i) {
+    const Seg &g = pieces[i];
+    uint64_t done = 0;
+    while (done < g.n) {
+      const ssize_t r = pwrite(fd, g.p + done, g.n - done, (off_t)(g.at + done));
+      if (r <= 0) {
+        werr = 1;
+        return;
+      }
+      done += (uint64_t)r;
+    }
+  }, 1);
+  const bool ok = (close(fd) == 0) && werr == 0;
+  if (!ok) return set_error(h, MH_ERR_ARG, std::string("mh_index_save_mhm: write failed: ") + path);
   return MH_OK;
 }
 
