@@ -97,10 +97,13 @@ def datasets():
            114, "n_runs")
 
 
+
+
 # Datasets whose .mhm file (the reference's own hashmap database, written by
 # SaveHashmapTableV3 and loaded back before classification) is kept as a
 # fixture too: ref_<name>.mhm.gz.
 MHM_FIXTURES = ("t22_tiny_mixed", "t18_tiny_mixed", "t30_tiny_nruns")
+LOG_KINDS = ("Raw", "HighSensitivity", "HighPrecision", "Balanced")
 
 
 COMP = np.zeros(256, np.uint8)
@@ -249,9 +252,14 @@ def run_reference(workdir, threshold):
             if fn == "results_Raw.log":
                 outs.append(os.path.join(root, fn))
     assert len(outs) == 1, outs
-    mhm = [os.path.join(os.path.dirname(outs[0]), "hashmap_reference.mhm")]
+    outdir = os.path.dirname(outs[0])
+    mhm = [os.path.join(outdir, "hashmap_reference.mhm")]
     assert os.path.exists(mhm[0]), mhm
-    res = {"_mhm": mhm[0]}
+    logs = {}
+    for kind in LOG_KINDS:
+        with open(os.path.join(outdir, f"results_{kind}.log"), "rb") as f:
+            logs[kind] = f.read().decode()
+    res = {"_mhm": mhm[0], "_logs": logs}
     with open(outs[0]) as f:
         next(f)
         for line in f:
@@ -287,7 +295,8 @@ def main():
         r1 = run_reference(os.path.join(base, "a"), T)
         r2 = run_reference(os.path.join(base, "b"), T)
         mhm = [open(r.pop("_mhm"), "rb").read() for r in (r1, r2)]
-        if r1 != r2 or mhm[0] != mhm[1]:
+        logs = [r.pop("_logs") for r in (r1, r2)]
+        if r1 != r2 or mhm[0] != mhm[1] or logs[0] != logs[1]:
             print(f"{name}: reference runs disagree, fixture not written")
             continue
         if name in MHM_FIXTURES:
@@ -301,6 +310,8 @@ def main():
             "reads": [qb[int(qo[i]):int(qo[i + 1])].tobytes().decode()
                       for i in range(len(qo) - 1)],
             "expected": {str(k): v for k, v in sorted(r1.items())},
+            # the four result logs as the reference wrote them
+            "logs": logs[0],
         }
         with gzip.GzipFile(os.path.join(OUT_DIR, f"ref_{name}.json.gz"), "wb", mtime=0) as f:
             f.write(json.dumps(fx).encode())
