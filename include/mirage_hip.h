@@ -260,6 +260,37 @@ mh_status mh_score_seeds(mh_handle *h, const mh_index *idx, const uint8_t *d_que
                          float *d_score, float *d_rel, uint64_t *d_sum_r, uint64_t *d_sum_c,
                          uint32_t *d_hits);
 
+/* ---- result logs ---------------------------------------------------------- */
+
+/* Score thresholds of the filtered logs (the LogThreshold_* settings,
+ * MirAge/LogManager.cpp:10-12; NULL selects 0.4, 1, 0.7). */
+typedef struct {
+  float high_sensitivity;  /* results_HighSensitivity.log */
+  float high_precision;    /* results_HighPrecision.log */
+  float balanced;          /* results_Balanced.log */
+} mh_log_thresholds;
+
+#define MH_RESULTS_APPEND 1u      /* append to the files, no header line (a later batch) */
+#define MH_RESULTS_ALL_GROUPS 2u  /* a line per score group (output_firstresults_only: 0) */
+
+/* Writes the reference's four result logs (results_Raw.log,
+ * results_HighSensitivity.log, results_HighPrecision.log, results_Balanced.log)
+ * into directory `dir`, replacing Analyser::LogMatrixResults /
+ * LogResultsToFileManager (MirAge/Analyser.cpp:226-491) and LogManager's
+ * files (MirAge/LogManager.cpp:294-404).  Host only.  Read r of the batch
+ * has InputID first_read_id + r; h_results / h_top_pool are a classify
+ * call's outputs; h_cands (may be NULL) its candidate output, which fixes
+ * the order of tied references exactly (without it, the top ties stand in
+ * for the scored candidates).  Headers are as MirAge keeps them ('>' name
+ * NUL; the last byte is not printed); h_taxids NULL means -1 throughout. */
+mh_status mh_write_results(const char *dir, uint64_t first_read_id, uint32_t n_reads,
+                           const uint8_t *h_read_headers, const uint64_t *h_read_header_off,
+                           const mh_read_result *h_results, const uint32_t *h_top_pool,
+                           const mh_cand *h_cands, uint64_t n_cands, uint32_t n_refs,
+                           const uint8_t *h_ref_headers, const uint64_t *h_ref_header_off,
+                           const int32_t *h_taxids, const mh_log_thresholds *thresholds,
+                           uint32_t flags);
+
 /* ---- batched form and tuning -------------------------------------------- */
 
 /* Tile choice for the length-bucketed persistent form. */

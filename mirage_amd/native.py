@@ -48,6 +48,15 @@ class MhmInfo(ctypes.Structure):
                 ("streampos_bytes", ctypes.c_uint32)]
 
 
+class LogThresholds(ctypes.Structure):
+    _fields_ = [("high_sensitivity", ctypes.c_float), ("high_precision", ctypes.c_float),
+                ("balanced", ctypes.c_float)]
+
+
+MH_RESULTS_APPEND = 1
+MH_RESULTS_ALL_GROUPS = 2
+
+
 class Counters(ctypes.Structure):
     _fields_ = [(f, ctypes.c_uint64) for f in COUNTER_FIELDS]
 
@@ -82,7 +91,7 @@ EXPORTS = ("mh_open", "mh_close", "mh_last_error", "mh_stream", "mh_max_read_len
            "mh_score_seeds", "mh_set_tiles", "mh_get_tiles", "mh_autotune", "mh_server_start",
            "mh_server_submit", "mh_server_wait", "mh_server_stop", "mh_mhm_open", "mh_mhm_close",
            "mh_mhm_get_info", "mh_mhm_references", "mh_mhm_bucket", "mh_index_from_mhm",
-           "mh_index_save_mhm")
+           "mh_index_save_mhm", "mh_write_results")
 
 _lib = None
 
@@ -134,6 +143,8 @@ def load():
         "mh_mhm_bucket": (i32, [P, u32, u32, P, P, u64, ctypes.POINTER(ctypes.c_int64)]),
         "mh_index_from_mhm": (i32, [P, P, i32, u32, ctypes.POINTER(P)]),
         "mh_index_save_mhm": (i32, [P, P, ctypes.c_char_p, P, P, P, u32, u32]),
+        "mh_write_results": (i32, [ctypes.c_char_p, u64, u32, P, P, P, P, P, u64, u32, P, P, P,
+                                   ctypes.POINTER(LogThresholds), u32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name, None)
@@ -344,6 +355,35 @@ class DeviceIndex:
                              ctypes.byref(cn) if wait else None)
         self.handle.check(st, "mh_classify")
         return (tn.value, cn.value) if wait else None
+
+
+def _headers(hs):
+    """Concatenated header bytes and n+1 offsets from a list of bytes."""
+    b = np.frombuffer(b"".join(hs) or b"\0", np.uint8)
+    off = np.zeros(len(hs) + 1, np.uint64)
+    off[1:] = np.cumsum([len(x) for x in hs])
+    return b, off
+
+
+def write_results(directory, out, top, read_headers, ref_headers, cands=None, taxids=None,
+                  first_read_id=0, thresholds=None, append=False, all_groups=False):
+    """The reference's four result logs for one batch (mh_write_results).
+    Headers as MirAge keeps them (b">name\0")."""
+    lib = load()
+    out = np.ascontiguousarray(out, dtype=READ_RESULT_DTYPE)
+    top = np.ascontiguousarray(top, dtype=np.uint32)
+    rh, rho = _headers(read_headers)
+    fh, fho = _headers(ref_headers)
+    c = None if cands is None else np.ascontiguousarray(cands, dtype=CAND_DTYPE)
+    tx = None if taxids is None else np.ascontiguousarray(taxids, dtype=np.int32)
+    thr = None if thresholds is None else ctypes.byref(LogThresholds(*thresholds))
+    flags = (MH_RESULTS_APPEND if append else 0) | (MH_RESULTS_ALL_GROUPS if all_groups else 0)
+    st = lib.mh_write_results(os.fsencode(str(directory)), first_read_id, len(out), _np_ptr(rh),
+                              _np_ptr(rho), _np_ptr(out), _np_ptr(top), _np_ptr(c),
+                              0 if c is None else len(c), len(ref_headers), _np_ptr(fh),
+                              _np_ptr(fho), _np_ptr(tx), thr, flags)
+    if st != MH_OK:
+        raise MirageError(st, last_error())
 
 
 def last_error():

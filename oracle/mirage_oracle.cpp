@@ -42,6 +42,7 @@
 #include <cmath>
 #include <cstring>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 namespace {
@@ -723,4 +724,43 @@ extern "C" int32_t mo_count_work(const mo_index *ix, const uint8_t *reads,
   }
   *work_out = sum;
   return 0;
+}
+
+/*
+ * The two library sorts the reference's result logs depend on, as GNU
+ * libstdc++ (the image's GCC 11) performs them, with the reference's
+ * comparators: candidates by relative score, descending
+ * (MirAge/Analyser.cpp:404), and a group's references by taxonomy
+ * occurrence (:252-293, not a strict weak ordering: id -1 goes first
+ * against anything).  They pin the Python restatement in oracle/stdsort.py.
+ */
+extern "C" void mo_std_sort_desc(float *score, uint32_t *id, uint64_t n) {
+  struct RS {  // MatrixResult::ResultsScores (MatrixResult.h:30-38)
+    float score;
+    uint64_t referenceID;
+  };
+  std::vector<RS> v(n);
+  for (uint64_t i = 0; i < n; i++) v[i] = RS{score[i], id[i]};
+  std::sort(v.begin(), v.end(), [](const RS &l, const RS &r) { return l.score > r.score; });
+  for (uint64_t i = 0; i < n; i++) {
+    score[i] = v[i].score;
+    id[i] = (uint32_t)v[i].referenceID;
+  }
+}
+
+extern "C" void mo_std_stable_sort_tax(uint64_t *ids, uint64_t n, const int32_t *taxids) {
+  std::vector<uint64_t> v(ids, ids + n);
+  std::unordered_map<uint64_t, size_t> occ;
+  for (uint64_t r : v) occ[(uint64_t)(int64_t)taxids[r]]++;
+  std::stable_sort(v.begin(), v.end(), [&](uint64_t a, uint64_t b) {
+    const uint64_t ta = (uint64_t)(int64_t)taxids[a], tb = (uint64_t)(int64_t)taxids[b];
+    if (ta == (uint64_t)-1) return true;
+    if (tb == (uint64_t)-1) return false;
+    if (ta == tb) return false;
+    if (occ[ta] == occ[tb]) return false;
+    if (occ[ta] > occ[tb]) return true;
+    if (occ[ta] < occ[tb]) return false;
+    return ta < tb;
+  });
+  std::copy(v.begin(), v.end(), ids);
 }

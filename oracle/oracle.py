@@ -72,6 +72,10 @@ def lib():
                                      ctypes.c_int32, ctypes.c_uint32, P, P, P]
         L.mo_relative_score.restype = ctypes.c_float
         L.mo_relative_score.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        L.mo_std_sort_desc.restype = None
+        L.mo_std_sort_desc.argtypes = [P, P, ctypes.c_uint64]
+        L.mo_std_stable_sort_tax.restype = None
+        L.mo_std_stable_sort_tax.argtypes = [P, ctypes.c_uint64, P]
         L.mo_count_work.restype = ctypes.c_int32
         L.mo_count_work.argtypes = [P, P, P, ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32,
                                     ctypes.c_int32, ctypes.POINTER(Work)]
@@ -183,6 +187,23 @@ class Index:
         if rc != 0:
             raise ValueError("oracle count_work failed")
         return {f: getattr(w, f) for f in WORK_FIELDS}
+
+
+def std_sort_desc(scores):
+    """libstdc++ std::sort of (float32 score, id) pairs, score descending."""
+    sc = np.ascontiguousarray([s for s, _ in scores], dtype=np.float32)
+    ids = np.ascontiguousarray([i for _, i in scores], dtype=np.uint32)
+    lib().mo_std_sort_desc(_ptr(sc), _ptr(ids), len(sc))
+    return list(zip(sc.tolist(), ids.tolist()))
+
+
+def std_stable_sort_tax(ids, taxids):
+    """libstdc++ std::stable_sort of reference ids with the result logs'
+    taxonomy comparator."""
+    v = np.ascontiguousarray(ids, dtype=np.uint64)
+    t = np.ascontiguousarray(taxids, dtype=np.int32)
+    lib().mo_std_stable_sort_tax(_ptr(v), len(v), _ptr(t))
+    return v.tolist()
 
 
 def top_sets(out, top):

@@ -81,6 +81,7 @@ def datasets():
            D.ReadSpec(n=400, lengths=L(mode=2, a=100, b=400), sub_rate=0.01, rc_frac=0.5),
            110, "variant_copies")
     yield ("t30_engineered", 30, None, None, 111, "engineered")
+    yield ("t30_ties20", 30, None, None, 115, "ties20")
     yield ("t22_tiny_mixed", 22,
            D.RefSpec(n=8, lengths=L(mode=2, a=300, b=3000)),
            D.ReadSpec(n=250, lengths=L(mode=2, a=40, b=400), p_exact=0.2, p_random=0.1,
@@ -212,6 +213,41 @@ def engineered(seed, T=30):
     return np.concatenate(refs), ro, np.concatenate(reads), qo
 
 
+def ties20(seed):
+    """24 references of 1.5 kb: a 400-base segment copied unchanged into
+    references 0-19 and a 300-base one into 20-23.  Reads: 60 from the first
+    segment (20-way ties, so the reference sorts more than 16 scored
+    candidates), 40 from the second, 50 from anywhere."""
+    rng = np.random.default_rng(seed)
+    rand = lambda n: rng.choice(list(b"ACGT"), n).astype(np.uint8)  # noqa: E731
+    seg_a, seg_b = rand(400), rand(300)
+    refs = []
+    for r in range(24):
+        x = rand(1500)
+        seg = seg_a if r < 20 else seg_b
+        at = 200 + 37 * (r % 7)
+        x[at:at + len(seg)] = seg
+        refs.append(x)
+    ro = np.zeros(25, np.uint64)
+    ro[1:] = np.cumsum([len(r) for r in refs])
+    rb = np.concatenate(refs)
+    reads = []
+    for i in range(150):
+        if i < 60:
+            a = int(rng.integers(0, 250))
+            reads.append(seg_a[a:a + 150].copy())
+        elif i < 100:
+            a = int(rng.integers(0, 150))
+            reads.append(seg_b[a:a + 150].copy())
+        else:
+            r = int(rng.integers(0, 24))
+            a = int(ro[r]) + int(rng.integers(0, 1350))
+            reads.append(rb[a:a + 150].copy())
+    qo = np.zeros(len(reads) + 1, np.uint64)
+    qo[1:] = np.cumsum([len(r) for r in reads])
+    return rb, ro, np.concatenate(reads), qo
+
+
 def write_fasta(path, prefix, by, off):
     with open(path, "wb") as f:
         for i in range(len(off) - 1):
@@ -280,6 +316,8 @@ def main():
             continue
         if edits == "engineered":
             rb, ro, qb, qo = engineered(seed, T)
+        elif edits == "ties20":
+            rb, ro, qb, qo = ties20(seed)
         else:
             rb, ro = D.make_refs(rspec, seed)
             rng = np.random.default_rng(seed)
