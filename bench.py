@@ -53,7 +53,8 @@ def run_steps(torch, handle, ix, d_reads, d_off, n, flags, steps, warmup, native
     timed = not (flags & native.MH_CLASSIFY_GRAPH)
     for _ in range(steps):
         ix.classify_device(d_reads, d_off, n, d_out, d_top, top_cap,
-                           flags=flags | (native.MH_CLASSIFY_TIMING if timed else 0), wait=False)
+                           flags=flags | (native.MH_CLASSIFY_TIMING if timed else 0), wait=False,
+                           sync_inputs=False)
         if timed:
             timings.append(handle.timing())
     ev1.record(stream)

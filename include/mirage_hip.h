@@ -260,6 +260,26 @@ mh_status mh_score_seeds(mh_handle *h, const mh_index *idx, const uint8_t *d_que
                          float *d_score, float *d_rel, uint64_t *d_sum_r, uint64_t *d_sum_c,
                          uint32_t *d_hits);
 
+/* ---- FASTA ingest ---------------------------------------------------------- */
+
+/* Parses FASTA text already in device memory into sequences and headers on
+ * the device, as Sequences::ReadFastaSequencesFromMem reads a file
+ * (MirAge/Sequences.cpp:57-211, :283-562): reading starts at the first line
+ * beginning with '>'; headers are kept as MirAge keeps them ('>' name NUL,
+ * a final '\r' dropped) in d_hdr with n+1 offsets d_hdr_off; sequence
+ * lines, without a final '\r', are concatenated per record and uppercased
+ * (:40-52) into d_seq with n+1 offsets d_seq_off, ready for mh_classify.
+ * Sizes go to *n_records, *seq_bytes, *hdr_bytes; with any output NULL only
+ * the sizes are computed (seq_bytes <= n_bytes; hdr_bytes <= n_bytes + 1).
+ * A text with an empty line, a line starting with '@', a header without a
+ * sequence line or a NUL byte returns MH_ERR_ARG naming the line (the
+ * reference's reading of those is undefined).  Synchronises the handle's
+ * stream. */
+mh_status mh_fasta_parse(mh_handle *h, const uint8_t *d_text, uint64_t n_bytes, uint8_t *d_seq,
+                         uint64_t seq_cap, uint64_t *d_seq_off, uint8_t *d_hdr, uint64_t hdr_cap,
+                         uint64_t *d_hdr_off, uint64_t rec_cap, uint64_t *n_records,
+                         uint64_t *seq_bytes, uint64_t *hdr_bytes);
+
 /* ---- result logs ---------------------------------------------------------- */
 
 /* Score thresholds of the filtered logs (the LogThreshold_* settings,

@@ -23,7 +23,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off", "-fno-fast-math"]
 
 HIP_SOURCES = ["mh_index.hip", "mh_classify.hip", "mh_api.hip", "mh_serve.hip",
-               "mh_autotune.hip", "mh_mhm.hip"]
+               "mh_autotune.hip", "mh_mhm.hip", "mh_fasta.hip"]
 HOST_SOURCES = ["matrix_hip.cpp", "mh_results.cpp"]
 
 

@@ -91,7 +91,7 @@ EXPORTS = ("mh_open", "mh_close", "mh_last_error", "mh_stream", "mh_max_read_len
            "mh_score_seeds", "mh_set_tiles", "mh_get_tiles", "mh_autotune", "mh_server_start",
            "mh_server_submit", "mh_server_wait", "mh_server_stop", "mh_mhm_open", "mh_mhm_close",
            "mh_mhm_get_info", "mh_mhm_references", "mh_mhm_bucket", "mh_index_from_mhm",
-           "mh_index_save_mhm", "mh_write_results")
+           "mh_index_save_mhm", "mh_write_results", "mh_fasta_parse")
 
 _lib = None
 
@@ -143,6 +143,8 @@ def load():
         "mh_mhm_bucket": (i32, [P, u32, u32, P, P, u64, ctypes.POINTER(ctypes.c_int64)]),
         "mh_index_from_mhm": (i32, [P, P, i32, u32, ctypes.POINTER(P)]),
         "mh_index_save_mhm": (i32, [P, P, ctypes.c_char_p, P, P, P, u32, u32]),
+        "mh_fasta_parse": (i32, [P, P, u64, P, u64, P, P, u64, P, u64, ctypes.POINTER(u64),
+                                 ctypes.POINTER(u64), ctypes.POINTER(u64)]),
         "mh_write_results": (i32, [ctypes.c_char_p, u64, u32, P, P, P, P, P, u64, u32, P, P, P,
                                    ctypes.POINTER(LogThresholds), u32]),
     }
@@ -154,6 +156,14 @@ def load():
         f.argtypes = args
     _lib = L
     return L
+
+
+def _torch_sync(t):
+    """Waits for torch's current stream on t's device, so that work torch
+    queued on t (a copy from the host, say) is complete before the library's
+    own stream reads it."""
+    import torch
+    torch.cuda.current_stream(t.device).synchronize()
 
 
 def _np_ptr(a):
@@ -215,6 +225,28 @@ class Handle:
             if f is not mhm:
                 f.close()
         return DeviceIndex.wrap(self, ix, threshold or info["threshold"], info["n_refs"])
+
+    def parse_fasta(self, d_text):
+        """FASTA text in a device tensor (uint8) to device tensors (sequence
+        bytes, n+1 offsets, header bytes, n+1 offsets) (mh_fasta_parse).
+        Torch's current stream is synchronised first (the library's stream
+        does not wait on it)."""
+        import torch
+        _torch_sync(d_text)
+        n = int(d_text.numel())
+        nr, ns, nh = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        self.check(self.lib.mh_fasta_parse(self.h, _dev_ptr(d_text), n, None, 0, None, None, 0,
+                                           None, 0, ctypes.byref(nr), ctypes.byref(ns),
+                                           ctypes.byref(nh)), "mh_fasta_parse")
+        seq = torch.empty(max(ns.value, 1), dtype=torch.uint8, device=d_text.device)
+        hdr = torch.empty(max(nh.value, 1), dtype=torch.uint8, device=d_text.device)
+        so = torch.empty(nr.value + 1, dtype=torch.int64, device=d_text.device)
+        ho = torch.empty(nr.value + 1, dtype=torch.int64, device=d_text.device)
+        self.check(self.lib.mh_fasta_parse(self.h, _dev_ptr(d_text), n, _dev_ptr(seq), ns.value,
+                                           _dev_ptr(so), _dev_ptr(hdr), nh.value, _dev_ptr(ho),
+                                           nr.value, ctypes.byref(nr), ctypes.byref(ns),
+                                           ctypes.byref(nh)), "mh_fasta_parse")
+        return seq[:ns.value], so, hdr[:nh.value], ho
 
     def set_tiles(self, tiles):
         arr = (Tile * len(tiles))(*[Tile(*t) for t in tiles])
@@ -345,8 +377,13 @@ class DeviceIndex:
         raise MirageError(MH_ERR_OVERFLOW, "output pools kept overflowing")
 
     def classify_device(self, d_reads, d_read_off, n_reads, d_out, d_top, top_cap, flags=0,
-                        d_cands=None, cand_cap=0, wait=True):
-        """Full path on device-resident torch tensors (mh_classify)."""
+                        d_cands=None, cand_cap=0, wait=True, sync_inputs=True):
+        """Full path on device-resident torch tensors (mh_classify).  The
+        library's stream does not wait on torch's: with sync_inputs the
+        current torch stream is synchronised first, so tensors just written
+        (a copy from the host, say) are complete."""
+        if sync_inputs:
+            _torch_sync(d_reads)
         lib, h = self.handle.lib, self.handle.h
         tn, cn = ctypes.c_uint64(), ctypes.c_uint64()
         st = lib.mh_classify(h, self.ix, _dev_ptr(d_reads), _dev_ptr(d_read_off), n_reads, flags,

@@ -58,3 +58,15 @@ def mhm_file(name, directory):
 def headers(n):
     """Reference headers as MirAge keeps them for the fixtures' FASTA names."""
     return [b">r%d\0" % i for i in range(n)]
+
+
+def fasta_headers(fx):
+    """(read headers, reference headers) as MirAge keeps them ('>' name
+    NUL): parsed from the dataset's FASTA texts when the fixture holds them,
+    else the generator's '>q<i>' and '>r<i>'."""
+    if "reference_fasta" in fx:
+        from oracle import fasta as F
+        return (F.parse(fx["query_fasta"].encode("latin-1"))[1],
+                F.parse(fx["reference_fasta"].encode("latin-1"))[1])
+    return ([b">q%d\0" % i for i in range(len(fx["reads"]))],
+            [b">r%d\0" % i for i in range(len(fx["refs"]))])

@@ -82,6 +82,11 @@ def datasets():
            110, "variant_copies")
     yield ("t30_engineered", 30, None, None, 111, "engineered")
     yield ("t30_ties20", 30, None, None, 115, "ties20")
+    yield ("t22_fasta_forms", 22,
+           D.RefSpec(n=6, lengths=L(mode=2, a=300, b=2500)),
+           D.ReadSpec(n=150, lengths=L(mode=2, a=40, b=400), p_exact=0.2, p_random=0.1,
+                      sub_rate=0.01, rc_frac=0.5),
+           116, "fasta_forms")
     yield ("t22_tiny_mixed", 22,
            D.RefSpec(n=8, lengths=L(mode=2, a=300, b=3000)),
            D.ReadSpec(n=250, lengths=L(mode=2, a=40, b=400), p_exact=0.2, p_random=0.1,
@@ -103,7 +108,7 @@ def datasets():
 # Datasets whose .mhm file (the reference's own hashmap database, written by
 # SaveHashmapTableV3 and loaded back before classification) is kept as a
 # fixture too: ref_<name>.mhm.gz.
-MHM_FIXTURES = ("t22_tiny_mixed", "t18_tiny_mixed", "t30_tiny_nruns")
+MHM_FIXTURES = ("t22_tiny_mixed", "t18_tiny_mixed", "t30_tiny_nruns", "t22_fasta_forms")
 LOG_KINDS = ("Raw", "HighSensitivity", "HighPrecision", "Balanced")
 
 
@@ -248,6 +253,26 @@ def ties20(seed):
     return rb, ro, np.concatenate(reads), qo
 
 
+def fasta_forms_text(prefix, by, off, rng, crlf):
+    """FASTA text the reference reads in its several forms: headers with
+    spaces and a '>' inside, line widths of 60, 61, 77 and unwrapped,
+    lowercase stretches, CRLF or LF line ends, and no newline after the last
+    line.  (The file must start with '>': Sequences.cpp:286-298.)"""
+    nl = b"\r\n" if crlf else b"\n"
+    out = bytearray()
+    for i in range(len(off) - 1):
+        seq = bytearray(by[int(off[i]):int(off[i + 1])].tobytes())
+        for _ in range(3):
+            a = int(rng.integers(0, len(seq)))
+            b = min(len(seq), a + int(rng.integers(5, 80)))
+            seq[a:b] = bytes(seq[a:b]).lower()
+        out += b">%s%d sample %d > part" % (prefix.encode(), i, i % 3) + nl
+        width = (60, 61, 77, 0)[i % 4] or len(seq)
+        for j in range(0, len(seq), width):
+            out += bytes(seq[j:j + width]) + nl
+    return bytes(out[:-len(nl)])
+
+
 def write_fasta(path, prefix, by, off):
     with open(path, "wb") as f:
         for i in range(len(off) - 1):
@@ -304,7 +329,7 @@ def run_reference(workdir, threshold):
                 continue
             cols = line.split(";")
             rid = int(cols[0])
-            refs = sorted(int(h.lstrip(">")[1:]) for h in cols[5].split(","))
+            refs = sorted(int(h.lstrip(">").split()[0][1:]) for h in cols[5].split(","))
             res[rid] = [cols[2], refs]
     return res
 
@@ -328,8 +353,17 @@ def main():
                 apply_edits(edits, rb, ro, qb, qo, rng)
         base = os.path.join(RUNS, name)
         os.makedirs(base, exist_ok=True)
-        write_fasta(os.path.join(base, "reference.fasta"), "r", rb, ro)
-        write_fasta(os.path.join(base, "query.fasta"), "q", qb, qo)
+        texts = None
+        if edits == "fasta_forms":
+            frng = np.random.default_rng(seed + 1)
+            texts = (fasta_forms_text("r", rb, ro, frng, True),
+                     fasta_forms_text("q", qb, qo, frng, False))
+            for fn, t in zip(("reference.fasta", "query.fasta"), texts):
+                with open(os.path.join(base, fn), "wb") as f:
+                    f.write(t)
+        else:
+            write_fasta(os.path.join(base, "reference.fasta"), "r", rb, ro)
+            write_fasta(os.path.join(base, "query.fasta"), "q", qb, qo)
         r1 = run_reference(os.path.join(base, "a"), T)
         r2 = run_reference(os.path.join(base, "b"), T)
         mhm = [open(r.pop("_mhm"), "rb").read() for r in (r1, r2)]
@@ -351,6 +385,9 @@ def main():
             # the four result logs as the reference wrote them
             "logs": logs[0],
         }
+        if texts is not None:  # the FASTA files as the reference read them
+            fx["reference_fasta"] = texts[0].decode("latin-1")
+            fx["query_fasta"] = texts[1].decode("latin-1")
         with gzip.GzipFile(os.path.join(OUT_DIR, f"ref_{name}.json.gz"), "wb", mtime=0) as f:
             f.write(json.dumps(fx).encode())
         print(f"{name}: {len(r1)} classified of {len(qo) - 1}")

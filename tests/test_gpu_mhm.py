@@ -95,7 +95,7 @@ def test_save_reproduces_reference_file(handle, tmp_path, name):
     ix = handle.build_index(rb, ro, fx["threshold"], last_bucket_empty=False)
     try:
         path = tmp_path / "saved.mhm"
-        ix.save_mhm(str(path), headers=_data.headers(len(fx["refs"])))
+        ix.save_mhm(str(path), headers=_data.fasta_headers(fx)[1])
         assert path.read_bytes() == _data.mhm_bytes(name)
     finally:
         ix.free()

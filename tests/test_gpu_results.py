@@ -14,13 +14,12 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("name", _data.golden_names())
 def test_logs_from_device_results(handle, tmp_path, name):
     fx, rb, ro, qb, qo = _data.load_golden(name)
-    n, n_refs = len(qo) - 1, len(ro) - 1
     ix = handle.build_index(rb, ro, fx["threshold"])
     try:
         for flags in (0, native.MH_CLASSIFY_BATCHED):
             out, top, cands = ix.classify_host(qb, qo, flags=flags, want_cands=True)
-            native.write_results(tmp_path, out, top, [b">q%d\0" % i for i in range(n)],
-                                 [b">r%d\0" % i for i in range(n_refs)], cands=cands)
+            qh, rh = _data.fasta_headers(fx)
+            native.write_results(tmp_path, out, top, qh, rh, cands=cands)
             for k in R.KINDS:
                 got = (tmp_path / f"results_{k}.log").read_text()
                 assert got == fx["logs"][k], (name, k, flags)

@@ -1,7 +1,7 @@
 """.mhm v4 hashmap files on the host: the oracle's restatement pinned against
 files the reference wrote, and the library's host reader against the oracle.
 
-The fixtures are the reference's own hashmap databases for three golden
+The fixtures are the reference's own hashmap databases for four golden
 datasets (tests/golden/make_ref_goldens.py): the reference wrote them with
 SaveHashmapTableV3 and classified the golden reads through them.  Their
 buckets must equal the oracle's index built from the same references, which
@@ -27,7 +27,7 @@ def oracle_index(name, last_bucket_empty):
 
 
 def test_fixtures_present():
-    assert NAMES == ["t18_tiny_mixed", "t22_tiny_mixed", "t30_tiny_nruns"]
+    assert NAMES == ["t18_tiny_mixed", "t22_fasta_forms", "t22_tiny_mixed", "t30_tiny_nruns"]
 
 
 @pytest.mark.parametrize("name", NAMES)
@@ -40,7 +40,7 @@ def test_reference_file_equals_oracle_index(name):
     fx, ix = oracle_index(name, False)
     assert m.threshold == fx["threshold"]
     assert m.refs == [r.encode() for r in fx["refs"]]
-    assert m.headers == _data.headers(len(fx["refs"]))
+    assert m.headers == _data.fasta_headers(fx)[1]
     assert m.taxids == [-1] * len(fx["refs"])
     assert len(m.buckets) == ix.num_buckets
     assert sum(len(v) for v in m.buckets.values()) == ix.num_entries

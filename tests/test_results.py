@@ -27,9 +27,14 @@ def classified(name):
     return fx, len(qo) - 1, len(ro) - 1, out, top, cands
 
 
-def oracle_logs(n, n_refs, out, cands, first_only=True, taxids=None):
-    rows = R.reads_from_oracle(out, cands, [">q%d" % i for i in range(n)])
-    return R.logs(rows, [">r%d" % i for i in range(n_refs)], taxids, first_only)
+def printed(hs):
+    return [h[:-1].decode("latin-1") for h in hs]
+
+
+def oracle_logs(fx, out, cands, first_only=True, taxids=None):
+    qh, rh = _data.fasta_headers(fx)
+    rows = R.reads_from_oracle(out, cands, printed(qh))
+    return R.logs(rows, printed(rh), taxids, first_only)
 
 
 def top_pool(out, top):
@@ -79,7 +84,7 @@ def test_stdsort_matches_libstdcxx():
 @pytest.mark.parametrize("name", NAMES)
 def test_oracle_logs_match_reference(name):
     fx, n, n_refs, out, top, cands = classified(name)
-    logs = oracle_logs(n, n_refs, out, cands)
+    logs = oracle_logs(fx, out, cands)
     for k in R.KINDS:
         assert logs[k] == fx["logs"][k], (name, k)
 
@@ -91,10 +96,10 @@ def test_planted_sort_mistakes_are_caught():
     orig_sort, orig_stable = stdsort.sort, stdsort.stable_sort
     try:
         stdsort.sort = lambda a, comp: a.sort(key=lambda t: -t[0])
-        assert oracle_logs(n, n_refs, out, cands)["Raw"] != fx["logs"]["Raw"]
+        assert oracle_logs(fx, out, cands)["Raw"] != fx["logs"]["Raw"]
         stdsort.sort = orig_sort
         stdsort.stable_sort = lambda a, comp: None
-        assert oracle_logs(n, n_refs, out, cands)["Raw"] != fx["logs"]["Raw"]
+        assert oracle_logs(fx, out, cands)["Raw"] != fx["logs"]["Raw"]
     finally:
         stdsort.sort, stdsort.stable_sort = orig_sort, orig_stable
 
@@ -106,9 +111,9 @@ def test_writer_matches_reference(name, tmp_path):
     unless they all tie), gives the reference's files byte for byte."""
     fx, n, n_refs, out, top, cands = classified(name)
     res, pool = top_pool(out, top)
+    qh, rh = _data.fasta_headers(fx)
     for with_cands in (True, False):
-        native.write_results(tmp_path, res, pool, headers("q", n), headers("r", n_refs),
-                             cands=cands if with_cands else None)
+        native.write_results(tmp_path, res, pool, qh, rh, cands=cands if with_cands else None)
         got = written(tmp_path)
         for k in R.KINDS:
             assert got[k] == fx["logs"][k], (name, k, with_cands)
