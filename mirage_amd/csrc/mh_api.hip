@@ -51,10 +51,15 @@ void exec_free(Exec &x) {
 mh_status ensure_scratch(mh_handle *h, Exec &x, size_t bytes) {
   if (x.scratch_bytes >= bytes) return MH_OK;
   MH_HIP(h, hipStreamSynchronize(x.stream));
+  // A captured call holds addresses inside the old arena: capture again.
+  if (x.graph) {
+    hipGraphExecDestroy(x.graph);
+    x.graph = nullptr;
+  }
+  const size_t want = std::max(bytes, x.scratch_bytes * 5 / 4);
   if (x.d_scratch) hipFree(x.d_scratch);
   x.d_scratch = nullptr;
   x.scratch_bytes = 0;
-  const size_t want = std::max(bytes, x.scratch_bytes * 5 / 4);
   MH_HIP(h, hipMalloc(&x.d_scratch, want));
   x.scratch_bytes = want;
   return MH_OK;

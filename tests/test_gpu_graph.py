@@ -51,3 +51,43 @@ def test_graph_replays_match_eager(handle):
     g = run(native.MH_CLASSIFY_BATCHED | native.MH_CLASSIFY_GRAPH, half)
     e = run(native.MH_CLASSIFY_BATCHED, half)
     assert np.array_equal(g[0], e[0]) and g[1] == e[1]
+
+
+def test_graph_captured_again_after_arena_grows():
+    """An eager call or a FASTA parse that grows the handle's scratch arena
+    frees the arena a captured call points into; the next graph call
+    captures again, and every graph call equals the eager one."""
+    import torch
+    h = native.Handle(0, 32768)
+    w = D.workload(3, 0.001)
+    w.refs.n = 200
+    rb, ro, qb, qo, _ = D.generate(w)
+    ix = h.build_index(rb, ro, w.threshold)
+    n = len(qo) - 1
+    small = n // 8
+    d_reads = torch.from_numpy(qb).cuda()
+    d_off = torch.from_numpy(qo.view(np.int64)).cuda()
+    cap = 16 * n + 64
+
+    def run(flags, nn):
+        d_out = torch.zeros(nn * native.READ_RESULT_DTYPE.itemsize, dtype=torch.uint8,
+                            device="cuda")
+        d_top = torch.zeros(cap * 4, dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        ix.classify_device(d_reads, d_off, nn, d_out, d_top, cap, flags=flags)
+        out = d_out.cpu().numpy().view(native.READ_RESULT_DTYPE)
+        top = d_top.cpu().numpy().view(np.uint32)
+        return out["highest"].view(np.uint32).tolist(), native.tie_sets(out, top)
+
+    graph = native.MH_CLASSIFY_BATCHED | native.MH_CLASSIFY_GRAPH
+    eager_small = run(native.MH_CLASSIFY_BATCHED, small)
+    assert run(graph, small) == eager_small
+    eager_all = run(native.MH_CLASSIFY_BATCHED, n)  # grows the arena
+    assert run(graph, small) == eager_small
+    text = b"".join(b">r%d\n" % i + b"ACGT" * 15 + b"\n" for i in range(400000))
+    seq, so, _, _ = h.parse_fasta(torch.from_numpy(np.frombuffer(text, np.uint8).copy()).cuda())
+    assert so.numel() == 400001 and seq.numel() == 400000 * 60
+    assert run(graph, small) == eager_small
+    assert run(graph, n) == eager_all
+    ix.free()
+    h.close()
