@@ -210,6 +210,39 @@ def mhm_leg(torch, native, handle, ix, info, rb, ro, threshold, d_reads, d_off, 
             "index_same_as_build": bool(same_index), "results_same": same_results}
 
 
+def fasta_leg(torch, native, handle, qb, qo, out, top, log):
+    """The reads as 61-column FASTA text in HBM, parsed on the device
+    (mh_fasta_parse) five times: time per parse and GB/s of text; the parsed
+    reads must equal the generated ones."""
+    n = len(qo) - 1
+    lens = np.diff(qo)
+    if not np.all(lens == lens[0]):
+        return {"skipped": "reads of one length only"}
+    m = int(lens[0])
+    seq = qb.reshape(n, m)
+    cuts = list(range(61, m, 61))
+    parts = np.split(seq, cuts, axis=1)
+    nl = np.full((n, 1), ord("\n"), np.uint8)
+    body = np.concatenate([x for p in parts for x in (p, nl)], axis=1)
+    heads = [b">read%d\n" % i for i in range(n)]
+    text = b"".join(h + body[i].tobytes() for i, h in enumerate(heads))
+    d_text = torch.from_numpy(np.frombuffer(text, np.uint8).copy()).cuda()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        pseq, pso, phdr, pho = handle.parse_fasta(d_text)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    same = bool(pseq.cpu().numpy().tobytes() == qb.tobytes() and pso.numel() == n + 1)
+    best = min(times)
+    log(f"fasta: {len(text) / 2**20:.0f} MiB of text, parse {best * 1e3:.2f} ms "
+        f"({len(text) / best / 1e9:.1f} GB/s), reads equal {same}")
+    return {"text_bytes": len(text), "records": n, "parse_ms": round(best * 1e3, 3),
+            "parse_ms_all": [round(x * 1e3, 3) for x in times],
+            "text_GBps": round(len(text) / best / 1e9, 2), "reads_equal": same}
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -282,6 +315,7 @@ def main():
     ap.add_argument("--mixed-reads", type=int, default=2_000_000)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-mhm", action="store_true")
+    ap.add_argument("--no-fasta", action="store_true")
     ap.add_argument("--serve-scale", type=float, default=1.0)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -397,6 +431,10 @@ def main():
             f"M reads/s, x{ps / bs:.2f}")
         del md_reads, md_off
 
+    fasta = None
+    if not args.no_fasta:
+        fasta = fasta_leg(torch, native, handle, qb, qo, out, top, log)
+
     mhm = None
     if not args.no_mhm:
         mhm = mhm_leg(torch, native, handle, ix, info, rb, ro, w.threshold, d_reads, d_off, n,
@@ -471,6 +509,7 @@ def main():
         "plain": plain,
         "mixed_length": mixed,
         "mhm_load": mhm,
+        "fasta_ingest": fasta,
         "served": serve,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -226,27 +226,46 @@ class Handle:
                 f.close()
         return DeviceIndex.wrap(self, ix, threshold or info["threshold"], info["n_refs"])
 
-    def parse_fasta(self, d_text):
+    def parse_fasta(self, d_text, exact=False):
         """FASTA text in a device tensor (uint8) to device tensors (sequence
         bytes, n+1 offsets, header bytes, n+1 offsets) (mh_fasta_parse).
         Torch's current stream is synchronised first (the library's stream
-        does not wait on it)."""
+        does not wait on it).  One call parses into buffers sized from the
+        text (sequence bytes never exceed it; headers and records are
+        guessed, and a second, exactly sized call follows when the guess is
+        short); exact=True asks for the sizes first instead."""
         import torch
         _torch_sync(d_text)
         n = int(d_text.numel())
+        dev = d_text.device
         nr, ns, nh = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
-        self.check(self.lib.mh_fasta_parse(self.h, _dev_ptr(d_text), n, None, 0, None, None, 0,
-                                           None, 0, ctypes.byref(nr), ctypes.byref(ns),
-                                           ctypes.byref(nh)), "mh_fasta_parse")
-        seq = torch.empty(max(ns.value, 1), dtype=torch.uint8, device=d_text.device)
-        hdr = torch.empty(max(nh.value, 1), dtype=torch.uint8, device=d_text.device)
-        so = torch.empty(nr.value + 1, dtype=torch.int64, device=d_text.device)
-        ho = torch.empty(nr.value + 1, dtype=torch.int64, device=d_text.device)
-        self.check(self.lib.mh_fasta_parse(self.h, _dev_ptr(d_text), n, _dev_ptr(seq), ns.value,
-                                           _dev_ptr(so), _dev_ptr(hdr), nh.value, _dev_ptr(ho),
-                                           nr.value, ctypes.byref(nr), ctypes.byref(ns),
-                                           ctypes.byref(nh)), "mh_fasta_parse")
-        return seq[:ns.value], so, hdr[:nh.value], ho
+
+        def call(seq, so, hdr, ho, caps):
+            return self.lib.mh_fasta_parse(
+                self.h, _dev_ptr(d_text), n, _dev_ptr(seq) if seq is not None else None, caps[0],
+                _dev_ptr(so) if so is not None else None,
+                _dev_ptr(hdr) if hdr is not None else None, caps[1],
+                _dev_ptr(ho) if ho is not None else None, caps[2],
+                ctypes.byref(nr), ctypes.byref(ns), ctypes.byref(nh))
+
+        if exact:
+            self.check(call(None, None, None, None, (0, 0, 0)), "mh_fasta_parse")
+            caps = (ns.value, nh.value, nr.value)
+        else:
+            caps = (n, n // 2 + 64, n // 32 + 16)
+        for attempt in range(2):
+            seq = torch.empty(max(caps[0], 1), dtype=torch.uint8, device=dev)
+            hdr = torch.empty(max(caps[1], 1), dtype=torch.uint8, device=dev)
+            so = torch.empty(caps[2] + 1, dtype=torch.int64, device=dev)
+            ho = torch.empty(caps[2] + 1, dtype=torch.int64, device=dev)
+            st = call(seq, so, hdr, ho, caps)
+            if st == MH_ERR_OVERFLOW and attempt == 0:
+                caps = (ns.value, nh.value, nr.value)
+                continue
+            self.check(st, "mh_fasta_parse")
+            break
+        r = nr.value
+        return seq[:ns.value], so[:r + 1], hdr[:nh.value], ho[:r + 1]
 
     def set_tiles(self, tiles):
         arr = (Tile * len(tiles))(*[Tile(*t) for t in tiles])

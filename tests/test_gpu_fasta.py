@@ -69,6 +69,33 @@ def test_parse_at_size(handle):
     assert [len(s) for s in seqs] == np.diff(qo).tolist()
 
 
+def test_parse_unaligned_and_resized(handle):
+    """A text that does not start on a 16-byte boundary (byte loads), and
+    texts whose records or headers outgrow the wrapper's first buffers (the
+    exactly sized second call), parse as the oracle does."""
+    import torch
+    rng = np.random.default_rng(11)
+    text = random_text(rng, 300, False, 61)
+    src = to_device(text)
+    big = torch.zeros(len(text) + 16, dtype=torch.uint8, device="cuda")
+    for off in (1, 3, 8):
+        view = big[off:off + len(text)]
+        view.copy_(src)
+        assert view.data_ptr() % 16 == off
+        seq, so, hdr, ho = handle.parse_fasta(view)
+        so, ho = so.cpu().numpy(), ho.cpu().numpy()
+        sb, hb = seq.cpu().numpy().tobytes(), hdr.cpu().numpy().tobytes()
+        got = ([sb[so[i]:so[i + 1]] for i in range(len(so) - 1)],
+               [hb[ho[i]:ho[i + 1]] for i in range(len(ho) - 1)])
+        assert got == F.parse(text), off
+    tiny = b"".join(b">%d\nA\n" % i for i in range(5000))  # about 8 bytes a record
+    assert device_parse(handle, tiny) == F.parse(tiny)
+    wide = b">" + b"h" * 9000 + b"\nAC\n>" + b"g" * 7000 + b"\nT"  # headers fill the text
+    assert device_parse(handle, wide) == F.parse(wide)
+    exact = handle.parse_fasta(to_device(tiny), exact=True)
+    assert exact[1].numel() == 5001
+
+
 @pytest.mark.parametrize("text,what", [
     (b">a\nACGT\n\nACGT\n", "empty line"),
     (b">a\nACGT\n@b\nACGT\n", "'@'"),
