@@ -463,7 +463,12 @@ mh_status mh_score_seeds(mh_handle *h, const mh_index *idx, const uint8_t *d_que
 }
 
 mh_status mh_set_tiles(mh_handle *h, const mh_tile *tiles, uint32_t n_tiles) {
-  if (!h || !tiles || !valid_tiles(tiles, n_tiles, h->max_read_len)) return MH_ERR_ARG;
+  if (!h) return MH_ERR_ARG;
+  if (!tiles || !valid_tiles(tiles, n_tiles, h->max_read_len))
+    return set_error(h, MH_ERR_ARG,
+                     "mh_set_tiles: 1-8 tiles of ascending max_len, the last at least "
+                     "max_read_len; block 8..1024 (a power of two); entries_cap a power of "
+                     "two >= 32");
   std::lock_guard<std::mutex> lk(h->mu);
   h->tiles.assign(tiles, tiles + n_tiles);
   h->tiles_version++;

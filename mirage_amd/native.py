@@ -12,7 +12,13 @@ import numpy as np
 from . import _build
 
 MH_OK = 0
+MH_ERR_ARG = -1
+MH_ERR_HIP = -2
+MH_ERR_NOMEM = -3
+MH_ERR_RANGE = -4
 MH_ERR_OVERFLOW = -5
+MH_ERR_BADBASE = -6
+MH_ERR_STATE = -7
 MH_INDEX_LAST_BUCKET_EMPTY = 1
 MH_CLASSIFY_ULTRAFAST = 1
 MH_CLASSIFY_NO_DIRECT = 2

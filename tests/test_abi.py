@@ -26,3 +26,16 @@ def test_library_exports_every_declared_symbol():
     lib = native.load()
     missing = [s for s in declared() if not hasattr(lib, s)]
     assert not missing, missing
+
+
+def test_python_constants_match_header():
+    """Every status, flag and result constant the wrapper defines has the
+    value of the header's #define of the same name."""
+    import re
+    from mirage_amd import native
+    defs = {m.group(1): int(m.group(2), 0) for m in
+            re.finditer(r"#define (MH_[A-Z0-9_]+) \(?(-?(?:0x[0-9a-fA-F]+|\d+))u?\)?", open(HEADER).read())}
+    names = [n for n in dir(native) if n.startswith("MH_") and isinstance(getattr(native, n), int)]
+    assert len(names) >= 12
+    for n in names:
+        assert n in defs and defs[n] == getattr(native, n), n

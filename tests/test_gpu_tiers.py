@@ -75,14 +75,15 @@ def test_repeat_reads_reach_every_tier(handle, flags):
 TILE_TABLES = {
     "block64": [(320, 64, 64, 0), (None, 64, 64, 0)],
     "lanes": [(200, 16, 32, 0), (600, 32, 32, 0), (None, 64, 64, 0)],
+    "lanes8": [(300, 8, 32, 0), (600, 32, 32, 0), (None, 64, 64, 0)],
 }
 
 
 @pytest.mark.parametrize("flags", [0, native.MH_CLASSIFY_BATCHED])
 @pytest.mark.parametrize("table", sorted(TILE_TABLES))
 def test_tiny_tiles_send_work_to_the_tiers(handle, flags, table):
-    """With small tiles (whole-workgroup teams, or lane teams of 16 and 32
-    lanes) many strands overflow; results are unchanged."""
+    """With small tiles (whole-workgroup teams, or lane teams of 8, 16 and
+    32 lanes) many strands overflow; results are unchanged."""
     saved = handle.get_tiles()
     try:
         handle.set_tiles([(ml or handle.max_read_len, b, e, g)
@@ -164,3 +165,18 @@ def test_preprobe_reads_at_unaligned_addresses(handle, shift):
     top = d_top.cpu().numpy().view(np.uint32)
     oout, otop, _ = O.Index(rb, ro, w.threshold).classify(qb, qo, top_cap=512)
     compare_reads(out, top, oout, otop, f"unaligned by {shift}")
+
+
+def test_invalid_tile_tables_are_refused(handle):
+    """A tile table the kernels cannot run is refused with its reason, and
+    the handle keeps its tiles."""
+    saved = handle.get_tiles()
+    bad = [[(handle.max_read_len, 16, 16, 0)],                      # entries_cap below 32
+           [(handle.max_read_len, 24, 32, 0)],                      # block not a power of two
+           [(400, 16, 32, 0), (200, 32, 64, 0)],                    # max_len not ascending
+           [(200, 16, 32, 0)]]                                      # last below max_read_len
+    for t in bad:
+        with pytest.raises(native.MirageError) as e:
+            handle.set_tiles(t)
+        assert e.value.status == native.MH_ERR_ARG and "mh_set_tiles" in str(e.value), t
+    assert handle.get_tiles() == saved
