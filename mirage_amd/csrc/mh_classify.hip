@@ -741,12 +741,6 @@ __device__ void filter_sorted(const Ws &ws, uint32_t E, uint32_t &nP) {
 
 enum : uint32_t { kSmallNo = 0, kSmallDirect, kSmallSorted };
 
-// Compare-exchange partner values for the register sort: (key, v) ordered
-// by key, then v.
-__device__ __forceinline__ bool elem_gt(uint64_t ka, uint32_t va, uint64_t kb, uint32_t vb) {
-  return ka > kb || (ka == kb && va > vb);
-}
-
 template <class TM>
 __device__ __forceinline__ uint64_t team_or64(uint64_t x) {
   constexpr int W = TM::kWorkerLanes;
@@ -755,9 +749,10 @@ __device__ __forceinline__ uint64_t team_or64(uint64_t x) {
   return x;
 }
 
-// Small working sets (at most 64 entries and 64 hits, a team inside one
-// wavefront) in registers: element i of the team's array is slot i & 1 of
-// lane i >> 1.  Gives the decisions of the LDS path below.
+// Small working sets (at most 64 entries and 64 hits, fewer than 2^26
+// references, a team inside one wavefront) in registers: element i of the
+// team's array is slot i & 1 of lane i >> 1.  Gives the decisions of the LDS
+// path below.
 //  * Direct hit: tested on the gathered entries with ballots (r* from the
 //    first bucket: as many entries as hits, all on its diagonal; no other
 //    reference of the second bucket present in every bucket 1..N-2).
@@ -775,7 +770,6 @@ __device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t 
   constexpr uint32_t W = TM::kWorkerLanes;
   static_assert(TM::kSize == W, "one-wavefront teams");
   const uint32_t lane = TM::rank();
-  uint64_t key[2];
   uint32_t ref[2], pos[2], qp[2], hit[2];
   bool ok[2];
 #pragma unroll
@@ -791,7 +785,6 @@ __device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t 
       pos[j] = ws.v1[e];
       qp[j] = ws.hit_j[hit[j]];
     }
-    key[j] = ((uint64_t)ref[j] << 32) | pos[j];
   }
   if (!(c.flags & MH_CLASSIFY_NO_DIRECT) && nhits == nprobe && nhits >= 3) {
     const uint32_t N = nhits;
@@ -846,41 +839,37 @@ __device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t 
       return kSmallDirect;
     }
   }
-  // Bitonic sort of next_pow2(E) elements by (key, v); v = qPos << 6 | hit
-  // (qPos orders v, as entries with equal keys differ in qPos).
-  uint32_t v[2];
+  // Bitonic sort of next_pow2(E) elements by (ref, refPos, qPos), packed
+  // into one u64 as ref << 38 | refPos << 6 | hit (hit order is qPos order;
+  // the caller guarantees ref < 2^26 and hit < 64, and no two entries are
+  // equal).  Padding is ~0, after every element.
+  uint64_t P[2];
 #pragma unroll
-  for (int j = 0; j < 2; j++) v[j] = ok[j] ? ((qp[j] << 6) | hit[j]) : 0xFFFFFFFFu;
+  for (int j = 0; j < 2; j++)
+    P[j] = ok[j] ? ((uint64_t)ref[j] << 38) | ((uint64_t)pos[j] << 6) | hit[j] : ~0ull;
   const uint32_t n2 = next_pow2(E < 2 ? 2 : E);
   for (uint32_t k = 2; k <= n2; k <<= 1) {
+    const bool asc = ((2 * lane) & k) == 0;
     for (uint32_t d = k >> 1; d > 1; d >>= 1) {
       const uint32_t lx = d >> 1;  // partner lane = lane ^ lx
-      const bool lower = (lane & lx) == 0;
-      const bool asc = ((2 * lane) & k) == 0;
+      // the lower lane of a pair keeps the minimum in an ascending run
+      const bool want_min = ((lane & lx) == 0) == asc;
       const uint32_t src = src_xor(lx);
 #pragma unroll
       for (int j = 0; j < 2; j++) {
-        const uint64_t pk = bperm64(src, key[j]);
-        const uint32_t pv = bperm(src, v[j]);
-        const bool pgt = elem_gt(pk, pv, key[j], v[j]);
-        const bool take = (lower == asc) ? !pgt && (pk != key[j] || pv != v[j]) : pgt;
-        if (take) {
-          key[j] = pk;
-          v[j] = pv;
-        }
+        const uint64_t pk = bperm64(src, P[j]);
+        P[j] = ((pk < P[j]) == want_min) ? pk : P[j];
       }
     }
     // d == 1: the two slots of a lane
-    const bool asc = ((2 * lane) & k) == 0;
-    if (elem_gt(key[0], v[0], key[1], v[1]) == asc) {
-      const uint64_t tk = key[0];
-      const uint32_t tv = v[0];
-      key[0] = key[1];
-      v[0] = v[1];
-      key[1] = tk;
-      v[1] = tv;
-    }
+    const uint64_t lo = P[0] < P[1] ? P[0] : P[1], hi = P[0] < P[1] ? P[1] : P[0];
+    P[0] = asc ? lo : hi;
+    P[1] = asc ? hi : lo;
   }
+  uint64_t key[2];  // ref << 32 | refPos
+#pragma unroll
+  for (int j = 0; j < 2; j++)
+    key[j] = P[j] == ~0ull ? ~0ull : ((P[j] >> 38) << 32) | ((P[j] >> 6) & 0xFFFFFFFFull);
   // Segment heads (a new reference), then each reference's hit mask: a
   // segmented OR over its elements, forward and backward.
   const uint32_t up1 = src_up<W>(1), down1 = src_down<W>(1);
@@ -890,7 +879,7 @@ __device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t 
 #pragma unroll
   for (int j = 0; j < 2; j++) {
     valid[j] = key[j] != ~0ull;
-    bit[j] = valid[j] ? (1ull << (v[j] & 63)) : 0ull;
+    bit[j] = valid[j] ? (1ull << (P[j] & 63)) : 0ull;
   }
   head[0] = lane == 0 || pk_hi != (uint32_t)(key[0] >> 32);
   head[1] = (key[0] >> 32) != (key[1] >> 32);
@@ -953,11 +942,11 @@ __device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t 
   TM::sync();
   if (keep0) {
     ws.k2[at] = key[0];
-    ws.v2[at] = v[0] >> 6;
+    ws.v2[at] = ws.hit_j[P[0] & 63];  // qPos of the hit
   }
   if (keep1) {
     ws.k2[at + (keep0 ? 1 : 0)] = key[1];
-    ws.v2[at + (keep0 ? 1 : 0)] = v[1] >> 6;
+    ws.v2[at + (keep0 ? 1 : 0)] = ws.hit_j[P[1] & 63];
   }
   nP = tot;
   TM::sync();
@@ -1145,7 +1134,7 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
   uint32_t nP = 0, nranks = 0, nchains = 0, G = 0;
   bool direct = false, sorted = false, small = false;
   if constexpr (BS <= 64) {
-    if (E <= 2 * BS && E <= 64 && nhits <= 64 && m < (1u << 26) && nph == 0) {
+    if (E <= 2 * BS && E <= 64 && nhits <= 64 && nph == 0 && ix.n_refs <= (1u << 26)) {
       small = true;
       const uint32_t r = small_path<TM>(c, ws, E, nhits, nprobe, nP);
       if (r == kSmallDirect) {
@@ -1157,8 +1146,8 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
       sorted = r == kSmallSorted;
     }
   }
-  // (qPos fits 26 bits in the register path; a strand that hit the copies of
-  // (0, 0) takes the general path, where their seeds are weighted.)
+  // (A strand that hit the copies of (0, 0) takes the general path, where
+  // their seeds are weighted.)
   if (!small && !(c.flags & MH_CLASSIFY_NO_DIRECT) && nhits == nprobe && nhits >= 3 &&
       nph == 0) {
     // Direct hit.  Every probe found a bucket, and one reference r* has
