@@ -427,6 +427,9 @@ __device__ __forceinline__ float matrix_score(uint64_t sum_r, uint64_t sum_c, ui
 
 // Head/tail extension of one seed by one seed worker (MatrixCPU.cpp:93-172).
 // Returns the matched length tot and the head length h; all lanes agree.
+// Tail and head are independent, so each round loads a chunk of both before
+// the ballots find the first mismatch: one memory round trip covers
+// kSeedLanes bases in each direction.
 template <class TM>
 __device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const uint8_t *rb,
                                             uint64_t ref_len, uint32_t sref, uint32_t rs,
@@ -439,41 +442,50 @@ __device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const 
   compares = 0;
   if (!ultrafast) {
     // Tail: at most min(T, ref_len - lref) bases, inside the query.
-    uint64_t tl = (uint64_t)lref < ref_len ? ref_len - lref : 0;
-    tl = tl < (uint64_t)T ? tl : (uint64_t)T;
+    uint64_t tl64 = (uint64_t)lref < ref_len ? ref_len - lref : 0;
+    tl64 = tl64 < (uint64_t)T ? tl64 : (uint64_t)T;
     const uint64_t qroom = re < m ? (uint64_t)(m - re) : 0;
-    tl = tl < qroom ? tl : qroom;
-    t = (uint32_t)tl;
-    for (uint32_t b = 0; b < tl; b += W) {
-      const uint32_t l = b + lane;
-      const bool mm = l < tl && q[re + l] != rb[(uint64_t)lref + l];
-      const unsigned long long bal = seed_ballot<TM>(mm);
-      if (bal) {
-        t = b + (uint32_t)__ffsll((long long)bal) - 1;
-        break;
-      }
-    }
-    // the reference compares until the first mismatch or the limit
-    compares = t + (t < tl ? 1u : 0u);
-    // Head: query row rs-1-l against reference byte sref-l; the base that
-    // meets a boundary is not counted.
+    const uint32_t tl = (uint32_t)(tl64 < qroom ? tl64 : qroom);
+    // Head: query row rs-1-l against reference byte sref-l for l <= lb; the
+    // base that meets a boundary is not counted (h stays lb without a
+    // mismatch).
+    uint32_t hl = 0, lb = 0;
     if (rs > 0) {
       const uint32_t hs = sref < (uint32_t)T ? sref : (uint32_t)T;
       if (hs > 0) {
-        const uint32_t lb = (rs - 1) < (hs - 1) ? (rs - 1) : (hs - 1);
-        h = lb;
-        for (uint32_t b = 0; b <= lb; b += W) {
-          const uint32_t l = b + lane;
-          const bool mm = l <= lb && q[rs - 1 - l] != rb[sref - l];
-          const unsigned long long bal = seed_ballot<TM>(mm);
-          if (bal) {
-            h = b + (uint32_t)__ffsll((long long)bal) - 1;
-            break;
-          }
-        }
-        compares += h + 1;
+        lb = (rs - 1) < (hs - 1) ? (rs - 1) : (hs - 1);
+        hl = lb + 1;
       }
     }
+    t = tl;
+    h = lb;
+    bool tdone = tl == 0, hdone = hl == 0;
+    for (uint32_t b = 0; !(tdone && hdone); b += W) {
+      const uint32_t l = b + lane;
+      const bool vt = !tdone && l < tl, vh = !hdone && l < hl;
+      const uint8_t rt = vt ? rb[(uint64_t)lref + l] : 0;  // both loads issue first
+      const uint8_t rh = vh ? rb[sref - l] : 0;
+      const unsigned long long bt = seed_ballot<TM>(vt && q[re + l] != rt);
+      const unsigned long long bh = seed_ballot<TM>(vh && q[rs - 1 - l] != rh);
+      if (!tdone) {
+        if (bt) {
+          t = b + (uint32_t)__ffsll((long long)bt) - 1;
+          tdone = true;
+        } else if (b + W >= tl) {
+          tdone = true;
+        }
+      }
+      if (!hdone) {
+        if (bh) {
+          h = b + (uint32_t)__ffsll((long long)bh) - 1;
+          hdone = true;
+        } else if (b + W >= hl) {
+          hdone = true;
+        }
+      }
+    }
+    // the reference compares until the first mismatch or the limit
+    compares = t + (t < tl ? 1u : 0u) + (hl ? h + 1 : 0u);
   }
   tot = t + h + mid;
   head = h;
