@@ -243,6 +243,100 @@ def fasta_leg(torch, native, handle, qb, qo, out, top, log):
             "text_GBps": round(len(text) / best / 1e9, 2), "reads_equal": same}
 
 
+def workload5_leg(torch, native, D, log, scale=1.0, batch=1_000_000):
+    """Workload 5 at full size (opt-in): 200,000 references of ~8 Gbp with
+    inserted repeat segments, k = 20 (T = 38), 10M x 150 bp reads.  The
+    index is built on the device; tiles are tuned on the first batch; the
+    tuned and default tiles must give the same results on it; then every
+    batch of `batch` reads is classified (batched form), timed end to end."""
+    w = D.workload(5, scale)
+    t = time.perf_counter()
+    rb, ro, qb, qo, _ = D.generate(w)
+    gen_s = time.perf_counter() - t
+    n = len(qo) - 1
+    log(f"workload 5: {len(rb) / 1e9:.2f} Gbp in {len(ro) - 1} refs, {n} reads "
+        f"(generated in {gen_s:.1f} s)")
+    h = native.Handle(0, 32768)
+    t = time.perf_counter()
+    ix = h.build_index(rb, ro, w.threshold)
+    build_s = time.perf_counter() - t
+    info = ix.info()
+    log(f"workload 5 index: {info['n_entries']:,} entries, {info['device_bytes'] / 2**30:.1f} GiB "
+        f"in HBM, built in {build_s:.1f} s (host copy included)")
+    del rb
+    d_reads, d_off = device_tensors(torch, qb, qo)
+    nb = (n + batch - 1) // batch
+    top_cap = 64 * batch + 1024
+    d_out = torch.empty(batch * native.READ_RESULT_DTYPE.itemsize, dtype=torch.uint8,
+                        device="cuda")
+    d_top = torch.empty(top_cap * 4, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+
+    def span(b):
+        lo = b * batch
+        return d_off[lo:min(lo + batch, n) + 1], min(batch, n - lo)
+
+    def classify(off_b, n_b):
+        # MH_ERR_OVERFLOW: the library grew its tie pool (or the output pool
+        # is short); the call is made again.
+        for attempt in range(4):
+            try:
+                return ix.classify_device(d_reads, off_b, n_b, d_out, d_top, top_cap,
+                                          flags=native.MH_CLASSIFY_BATCHED)
+            except native.MirageError as e:
+                if e.status != native.MH_ERR_OVERFLOW or attempt == 3:
+                    raise
+
+    def run0():
+        off0, n0 = span(0)
+        tn, _ = classify(off0, n0)
+        assert tn <= top_cap
+        out = d_out.cpu().numpy().view(native.READ_RESULT_DTYPE)[:n0].copy()
+        top = d_top.cpu().numpy().view(np.uint32).copy()
+        return out, native.tie_sets(out, top), h.counters()
+
+    default_tiles = h.get_tiles()
+    out_d, ties_d, ctr = run0()
+    t = time.perf_counter()
+    off0, n0 = span(0)
+    tiles = h.autotune(ix, d_reads, off0, n0, iters=1)
+    tune_s = time.perf_counter() - t
+    out_t, ties_t, _ = run0()
+    same = bool(np.array_equal(out_d["highest"].view(np.uint32), out_t["highest"].view(np.uint32))
+                and np.array_equal(out_d["strand"], out_t["strand"]) and ties_d == ties_t)
+    log(f"workload 5 tiles tuned in {tune_s:.1f} s: {tiles[:2]} (default {default_tiles[:2]}); "
+        f"results same {same}")
+    needed = 0
+    for b in range(nb):  # checked pass: the top-tie pool holds every batch
+        off_b, n_b = span(b)
+        tn, _ = classify(off_b, n_b)
+        needed = max(needed, tn)
+    assert needed <= top_cap
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for b in range(nb):
+        off_b, n_b = span(b)
+        ix.classify_device(d_reads, off_b, n_b, d_out, d_top, top_cap,
+                           flags=native.MH_CLASSIFY_BATCHED, wait=False, sync_inputs=False)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t
+    probes, entries = ctr["probes"], ctr["entries"]
+    log(f"workload 5: {n / sec / 1e6:.2f} M reads/s over {nb} batches of {batch} "
+        f"({sec * 1e3:.1f} ms), {entries / max(probes, 1):.1f} entries per probe, "
+        f"{ctr['overflow']} of {2 * n0} strands of batch 0 in the overflow tiers")
+    res = {"scale": scale, "reads": n, "refs": len(ro) - 1, "ref_bases": int(ro[-1]), "threshold": w.threshold,
+           "index_entries": info["n_entries"], "index_GiB": round(info["device_bytes"] / 2**30, 2),
+           "index_build_s": round(build_s, 2), "batch_reads": batch, "batches": nb,
+           "reads_per_s": round(n / sec, 1), "ms_total": round(sec * 1e3, 2),
+           "entries_per_probe": round(entries / max(probes, 1), 2),
+           "overflow_strands_batch0": ctr["overflow"], "tiles": [list(x) for x in tiles],
+           "default_tiles_same_results": same, "tune_s": round(tune_s, 1),
+           "top_pool_needed_max": int(needed)}
+    ix.free()
+    h.close()
+    return res
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -316,6 +410,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-mhm", action="store_true")
     ap.add_argument("--no-fasta", action="store_true")
+    ap.add_argument("--workload5", action="store_true",
+                    help="also run workload 5 at full size (200,000 refs, ~8 Gbp, 10M reads; "
+                         "about 80 GB of HBM, a few minutes)")
+    ap.add_argument("--workload5-scale", type=float, default=1.0)
     ap.add_argument("--serve-scale", type=float, default=1.0)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -431,6 +529,10 @@ def main():
             f"M reads/s, x{ps / bs:.2f}")
         del md_reads, md_off
 
+    wl5 = None
+    if args.workload5:
+        wl5 = workload5_leg(torch, native, D, log, args.workload5_scale)
+
     fasta = None
     if not args.no_fasta:
         fasta = fasta_leg(torch, native, handle, qb, qo, out, top, log)
@@ -510,6 +612,7 @@ def main():
         "mixed_length": mixed,
         "mhm_load": mhm,
         "fasta_ingest": fasta,
+        "workload5": wl5,
         "served": serve,
         "roofline": roofline,
         "cpu_baseline": cpu,

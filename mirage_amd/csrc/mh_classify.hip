@@ -326,6 +326,18 @@ struct Ws {
   uint32_t *red;       // kRedWords words: block-scan partials, then broadcasts
   unsigned long long *acc;  // 2
   unsigned long long *stats;  // kNumStats, per workgroup
+  struct RefSlot *slots;  // nslot references scored together
+  uint32_t *slot_bm;      // nslot row bitmaps of qcap / 32 + 1 words
+};
+
+// One reference of a multi-reference scoring batch.
+struct RefSlot {
+  unsigned long long acc[2];  // sumR, sumC
+  uint64_t ro, len;           // the reference's byte offset and length
+  uint32_t r, R;              // rank, reference id
+  uint32_t c0, nchain;        // its first chain and chain count
+  uint32_t nseed, base;       // seeds (chains + unstored copies of (0, 0)); first batch item
+  uint32_t pad[2];
 };
 
 __host__ __device__ inline size_t al8(size_t x) { return (x + 7) & ~size_t(7); }
@@ -343,6 +355,7 @@ __host__ __device__ inline size_t ws_bytes(const Caps &c) {
   b += al8(c.ecap * 16ull);
   b += al8((c.qcap / 32 + 1) * 4ull);
   b += kRedWords * 4 + 16 + 64;
+  b += c.nslot * (sizeof(RefSlot) + (c.qcap / 32 + 1) * 4ull);
   return b;
 }
 
@@ -375,6 +388,8 @@ __device__ inline Ws ws_carve(unsigned char *base, const Caps &c) {
   w.red = (uint32_t *)take(kRedWords * 4);
   w.acc = (unsigned long long *)take(16);
   w.stats = (unsigned long long *)take(64);
+  w.slots = (RefSlot *)take(c.nslot * sizeof(RefSlot));
+  w.slot_bm = (uint32_t *)take(c.nslot * (c.qcap / 32 + 1) * 4ull);
   return w;
 }
 
@@ -549,6 +564,167 @@ __device__ void score_ref(const uint8_t *q, uint32_t m, const uint8_t *rb, uint6
   TM::sync();
   sum_r = ws.acc[0];
   sum_c = ws.acc[1];
+}
+
+// Scores every rank whose longest chain equals G, for teams of whole
+// workgroups: up to nslot references at a time get a slot (sums, row bitmap),
+// and the seeds of all of them are spread over the seed workers, so the
+// memory round trips of different references overlap.  Per reference the
+// sums, bitmap and score are those of score_ref (integer sums and a bitwise
+// OR do not depend on the order).  Sets rel_of / valid_of per rank (0 for
+// ranks not scored); seeds_total, seeded and lut_bad are meaningful on rank 0.
+template <class TM>
+__device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Caps &cap,
+                                            uint32_t m, uint32_t nranks, uint32_t nchains,
+                                            uint32_t G, uint32_t nhits, uint32_t nph,
+                                            bool ultrafast, uint32_t read, uint32_t strand,
+                                            unsigned long long &seeds_total, uint32_t &seeded,
+                                            uint32_t &lut_bad) {
+  constexpr uint32_t BS = TM::kSize, NW = BS / TM::kSeedLanes;
+  static_assert(BS >= 64 && BS % 64 == 0, "whole-workgroup teams");
+  const uint32_t tid = TM::rank(), wid = tid / TM::kSeedLanes, lane = seed_lane<TM>();
+  const IndexView &ix = c.ix;
+  const int T = ix.T, s = ix.s;
+  const uint32_t *cfl = (const uint32_t *)ws.k1;
+  float *rel_of = (float *)ws.t2;
+  uint32_t *valid_of = ws.t1;
+  const uint32_t bw = cap.qcap / 32 + 1, nw = (m + 31) >> 5;
+  seeds_total = 0;
+  seeded = 0;
+  for (uint32_t r = tid; r < nranks; r += BS) valid_of[r] = 0;
+  if (tid == 0) ws.red[kBcast + 3] = 0;  // a relative score outside the tables
+  for (uint32_t base = 0; base < nranks; base += BS) {
+    const uint32_t r = base + tid;
+    const bool f = r < nranks && ws.t3[r] == G;
+    uint32_t tot;
+    const uint32_t idx = TM::excl_scan(f ? 1u : 0u, ws.red, tot);
+    for (uint32_t g = 0; g < tot; g += cap.nslot) {
+      const uint32_t nb = min(cap.nslot, tot - g);
+      TM::sync();
+      if (f && idx >= g && idx < g + nb) {
+        RefSlot &sl = ws.slots[idx - g];
+        const uint32_t R = ws.rref[r];
+        const uint64_t ro = ix.ref_off[R];
+        const uint32_t c0 = ws.v1[r];
+        const uint32_t c1 = (r + 1 < nranks) ? ws.v1[r + 1] : nchains;
+        // reference 0 also gets the unstored copies of (0, 0) as one-pair
+        // seeds, one per probe that hit bucket 0 (Analyser.cpp:2421-2481)
+        const uint32_t nvirt = R == 0 ? nph : 0u;
+        sl.acc[0] = 0;
+        sl.acc[1] = 0;
+        sl.ro = ro;
+        sl.len = ix.ref_off[R + 1] - ro;
+        sl.r = r;
+        sl.R = R;
+        sl.c0 = c0;
+        sl.nchain = c1 - c0;
+        sl.nseed = c1 - c0 + nvirt;
+      }
+      for (uint32_t w = tid; w < nb * bw; w += BS) ws.slot_bm[w] = 0;
+      TM::sync();
+      if (tid == 0) {
+        uint32_t acc = 0;
+        for (uint32_t b = 0; b < nb; b++) {
+          ws.slots[b].base = acc;
+          acc += ws.slots[b].nseed;
+          seeds_total += ws.slots[b].nchain +
+                         (unsigned long long)(ws.slots[b].nseed - ws.slots[b].nchain) * ix.phantom;
+        }
+        seeded += nb;
+        ws.red[kBcast + 2] = acc;
+      }
+      TM::sync();
+      const uint32_t nitems = ws.red[kBcast + 2];
+      unsigned long long ncmp = 0;
+      for (uint32_t i = wid; i < nitems; i += NW) {
+        uint32_t b = 0;  // the slot of item i
+        while (b + 1 < nb && ws.slots[b + 1].base <= i) b++;
+        const RefSlot &sl = ws.slots[b];
+        const uint32_t jj = i - sl.base;
+        uint32_t sref, sq, lref, lq, wt;
+        if (jj < sl.nchain) {
+          const uint32_t cid = sl.c0 + jj;
+          const uint32_t f0 = cfl[2 * cid], l0 = cfl[2 * cid + 1];
+          sref = (uint32_t)ws.k2[f0];
+          sq = ws.v2[f0];
+          lref = (uint32_t)ws.k2[l0] + (uint32_t)s;  // Analyser.cpp:2465-2467
+          lq = ws.v2[l0] + (uint32_t)s;
+          wt = 1;
+        } else {
+          uint32_t v = jj - sl.nchain, j = 0;
+          for (uint32_t t = 0; t < nhits; t++)
+            if (ws.hit_lo[t] == 0 && ws.hit_base[t + 1] > ws.hit_base[t]) {
+              if (v == 0) {
+                j = ws.hit_j[t];
+                break;
+              }
+              v--;
+            }
+          sref = 0;
+          sq = j;
+          lref = (uint32_t)s;
+          lq = j + (uint32_t)s;
+          wt = ix.phantom;
+        }
+        const uint32_t rs = sq, re = lq;
+        if (rs > re) continue;  // MatrixCPU.cpp:57-58
+        uint32_t tl, h, cmp;
+        extend_seed<TM>(ws.q, m, ix.bytes + sl.ro, sl.len, sref, rs, lref, re, T, ultrafast, tl, h,
+                        cmp);
+        ncmp += (unsigned long long)wt * cmp;
+        if (tl >= (uint32_t)T) {  // MatrixCPU.cpp:178-184
+          if (lane == 0) {
+            atomicAdd(&ws.slots[b].acc[0],
+                      (unsigned long long)wt * ((unsigned long long)tl * (tl + 1) / 2));
+            atomicAdd(&ws.slots[b].acc[1], (unsigned long long)wt * tl);
+          }
+          set_bits<TM>(ws.slot_bm + b * bw, rs - h, tl);
+        }
+      }
+      if (lane == 0 && ncmp && ws.stats) atomicAdd(&ws.stats[kExt], ncmp);
+      TM::sync();
+      // one wavefront per slot: row hits, Score, relative score
+      const uint32_t wave = tid >> 6, l64 = tid & 63;
+      for (uint32_t b = wave; b < nb; b += BS / 64) {
+        const uint32_t *bm = ws.slot_bm + b * bw;
+        uint32_t pc = 0;
+        for (uint32_t w = l64; w < nw; w += 64) pc += __popc(bm[w]);
+        const uint32_t hits = wave_sum(pc);
+        if (l64 == 0) {
+          const RefSlot &sl = ws.slots[b];
+          const float score = matrix_score(sl.acc[0], sl.acc[1], hits, m);
+          float rel = -1.0f;
+          uint32_t valid = 0;
+          if (score > -1.0f) {  // MatrixCPU.cpp:356
+            if (rel_score(c.lut, m, sl.len, score, rel)) valid = 1;
+            else ws.red[kBcast + 3] = 1;
+          }
+          rel_of[sl.r] = rel;
+          valid_of[sl.r] = valid;
+          if (c.out.cands) {
+            const unsigned long long slot = atomicAdd(&c.out.ctr->cand_used, 1ull);
+            if (slot < c.out.cand_cap) {
+              mh_cand cd;
+              cd.read = read;
+              cd.strand = strand;
+              cd.ref = sl.R;
+              cd.hits = hits;
+              cd.sum_r = sl.acc[0];
+              cd.sum_c = sl.acc[1];
+              cd.score = score;
+              cd.rel = rel;
+              cd.n_seeds = (uint32_t)(sl.nchain +
+                                      (unsigned long long)(sl.nseed - sl.nchain) * ix.phantom);
+              cd.pad = 0;
+              c.out.cands[slot] = cd;
+            }
+          }
+        }
+      }
+    }
+  }
+  TM::sync();
+  lut_bad = ws.red[kBcast + 3];
 }
 
 template <class TM>
@@ -968,8 +1144,9 @@ __device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t 
 // Processes query q.  Returns kOverflow (uniformly) when the working set
 // exceeds the workspace capacities; nothing is written in that case.
 template <class TM>
-__device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, uint32_t q,
-                                  bool last_resort, const uint8_t *comp) {
+__device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap,
+                                                  uint32_t q, bool last_resort,
+                                                  const uint8_t *comp) {
   constexpr uint32_t BS = TM::kSize;
   const uint32_t tid = TM::rank();
   const uint32_t read = q >> 1, strand = q & 1;
@@ -1352,7 +1529,17 @@ __device__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap, u
   uint32_t *valid_of = ws.t1;
   unsigned long long seeds_total = 0;
   uint32_t seeded = 0;
-  for (uint32_t r = 0; r < nranks; r++) {
+  bool multi = false;
+  if constexpr (TM::kPerWave == 1) {
+    if (cap.nslot != 0) {
+      multi = true;
+      uint32_t lut_bad = 0;
+      score_multi<TM>(c, ws, cap, m, nranks, nchains, G, nhits, nph, ultrafast, read, strand,
+                      seeds_total, seeded, lut_bad);
+      if (lut_bad) res.status |= MH_READ_CAPACITY;
+    }
+  }
+  for (uint32_t r = 0; r < nranks && !multi; r++) {
     if (ws.t3[r] != G) {
       if (tid == 0) valid_of[r] = 0;
       continue;
@@ -1977,8 +2164,13 @@ __global__ void k_max_len(Reads rd, unsigned long long *out) {
 // first the entry capacity (to what fits, multiples of 16, down to 256),
 // then the query length.  Queries beyond the capacities go to the overflow
 // tiers.
+// References scored together by teams of 128 or more threads (and the HBM
+// tiers): each takes a 64-byte slot and a row bitmap.
+constexpr uint32_t kRefSlots = 16;
+
 Caps caps_for(const mh_tile &t, int s, size_t limit) {
   Caps c;
+  c.nslot = t.block >= 128 ? kRefSlots : 0;
   const uint32_t sd = (uint32_t)(s > 0 ? s : 1);
   c.qcap = (t.max_len + 63) & ~63u;
   c.pcap = c.qcap / sd + 2;
@@ -2020,6 +2212,7 @@ size_t tile_lds(const mh_tile &t, const Caps &c) { return teams_of(t.block) * ws
 
 Caps hbm_caps(uint32_t max_read_len, int s, uint32_t ecap) {
   Caps c;
+  c.nslot = kRefSlots;
   c.qcap = (max_read_len + 63) & ~63u;
   c.pcap = c.qcap / (uint32_t)(s > 0 ? s : 1) + 2;
   c.ecap = ecap;

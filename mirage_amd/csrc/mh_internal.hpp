@@ -99,6 +99,7 @@ struct Caps {
   uint32_t qcap;   // query bytes
   uint32_t pcap;   // probe hits
   uint32_t ecap;   // bucket entries (power of two)
+  uint32_t nslot = 0;  // references scored together (workgroup teams); 0: one at a time
 };
 
 }  // namespace mh
