@@ -311,6 +311,46 @@ __device__ void bitonic_sort(uint64_t *key, uint32_t *val, uint32_t n) {
 }
 
 
+// Sorts n distinct u64 keys in place: the rank of each key among the first
+// n by shuffles for one worker's worth, otherwise the network of
+// bitonic_sort without a value array.
+template <class TM>
+__device__ void bitonic_sort_keys(uint64_t *key, uint32_t n) {
+  constexpr uint32_t W = TM::kWorkerLanes;
+  if (n <= W) {
+    TM::sync();
+    if (TM::worker() == 0) {
+      const uint32_t lane = TM::wlane();
+      const uint64_t k = lane < n ? key[lane] : ~0ull;
+      uint32_t rank = 0;
+      for (uint32_t u = 0; u < n; u++) rank += bperm64(src_at<W>(u), k) < k ? 1u : 0u;
+      __builtin_amdgcn_wave_barrier();
+      if (lane < n) key[rank] = k;
+    }
+    TM::sync();
+    return;
+  }
+  const uint32_t n2 = next_pow2(n);
+  for (uint32_t size = 2; size <= n2; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      TM::sync();
+      const uint32_t lg = 31 - __clz(stride);
+      const bool mirror = stride == (size >> 1);
+      for (uint32_t t = TM::rank(); t < (n2 >> 1); t += TM::kSize) {
+        const uint32_t i = ((t >> lg) << (lg + 1)) + (t & (stride - 1));
+        const uint32_t j = mirror ? (i ^ (size - 1)) : (i + stride);
+        if (j >= n) continue;  // j > i: +infinity at j, nothing moves
+        const uint64_t ka = key[i], kb = key[j];
+        if (kb < ka) {
+          key[i] = kb;
+          key[j] = ka;
+        }
+      }
+    }
+  }
+  TM::sync();
+}
+
 // Workspace carved from LDS (or HBM for the large kernel).
 struct Ws {
   uint8_t *q;          // qcap query bases
@@ -883,6 +923,64 @@ __device__ void filter_hashed(const Ws &ws, uint32_t E, uint32_t ecap, uint32_t 
   TM::sync();
 }
 
+// Successive-occurrence filter for at most 64 hits: each reference's slot in
+// an LDS hash map collects the mask of the hits it occurs in (a repeat of one
+// hit sets one bit, Analyser.cpp:2057-2058); its longest run of set bits is
+// its longest run of consecutive hit indices.  Same decisions as
+// filter_hashed without its sort.  Kept pairs go to k2 packed as
+// ref << 38 | refPos << 6 | hit, unsorted.  Needs next_pow2(2E) <= ecap (the
+// masks take t1..t2, the keys t3, the runs rref) and references < 2^26.
+template <class TM>
+__device__ void filter_masked(const Ws &ws, uint32_t E, uint32_t &nP) {
+  constexpr uint32_t BS = TM::kSize;
+  const uint32_t tid = TM::rank();
+  const uint32_t cap = next_pow2(2 * E), mask = cap - 1;
+  unsigned long long *hm = reinterpret_cast<unsigned long long *>(ws.t1);
+  uint32_t *rkeys = ws.t3, *run = ws.rref;
+  for (uint32_t i = tid; i < cap; i += BS) {
+    rkeys[i] = kEmpty;
+    hm[i] = 0;
+  }
+  TM::sync();
+  for (uint32_t e = tid; e < E; e += BS) {
+    const uint64_t k = ws.k1[e];
+    const uint32_t slot = map_slot(rkeys, mask, (uint32_t)(k >> 32));
+    atomicOr(&hm[slot], 1ull << ((uint32_t)k & 63));
+    ws.v2[e] = slot;
+  }
+  TM::sync();
+  uint32_t mr = 0;
+  for (uint32_t i = tid; i < cap; i += BS) {
+    uint64_t x = hm[i];
+    uint32_t r = 0;
+    while (x) {
+      x &= x >> 1;
+      r++;
+    }
+    run[i] = r;
+    mr = max(mr, r);
+  }
+  const uint32_t maxrun = TM::max(mr, ws.red);
+  const uint32_t cut = maxrun >= 2 ? maxrun - 1 : 1;
+  uint32_t nkeep = 0;
+  for (uint32_t base = 0; base < E; base += BS) {
+    const uint32_t e = base + tid;
+    uint32_t keep = 0;
+    uint64_t pk = 0;
+    if (e < E) {
+      const uint64_t k = ws.k1[e];
+      keep = run[ws.v2[e]] >= cut ? 1u : 0u;
+      pk = ((k >> 32) << 38) | ((uint64_t)ws.v1[e] << 6) | ((uint32_t)k & 63);
+    }
+    uint32_t tot;
+    const uint32_t ex = TM::excl_scan(keep, ws.red, tot);
+    if (keep) ws.k2[nkeep + ex] = pk;
+    nkeep += tot;
+  }
+  nP = nkeep;
+  TM::sync();
+}
+
 // Successive-occurrence filter by sorting every pair by (ref, hit); used
 // when (ref, hit) does not pack into 32 bits.
 template <class TM>
@@ -1337,8 +1435,12 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
   }
   // (A strand that hit the copies of (0, 0) takes the general path, where
   // their seeds are weighted.)
+  // (Tried only while buckets average at most 4 entries: the test scans
+  // every bucket per entry of the first, and with large buckets one
+  // reference alone in every bucket is rare; the full path below gives the
+  // same result either way.)
   if (!small && !(c.flags & MH_CLASSIFY_NO_DIRECT) && nhits == nprobe && nhits >= 3 &&
-      nph == 0) {
+      nph == 0 && E <= 4 * nhits) {
     // Direct hit.  Every probe found a bucket, and one reference r* has
     // exactly one entry in each bucket, all on one diagonal.  Then r* has the
     // longest run N = nhits, the cut is N - 1, and if no other reference
@@ -1441,13 +1543,26 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
     // pairs in k2/v2 as (ref << 32 | refPos, qPos), unsorted.
     // (Done in registers by small_path for small working sets.)
     if (!sorted) {
-      if (nhits < (1u << 12) && ix.n_refs < (1u << 20))
-        filter_hashed<TM>(ws, E, cap.ecap, nP);
-      else
-        filter_sorted<TM>(ws, E, nP);
-      // Sort by (reference, refPos, qPos) (Analyser.cpp:1024-1059).
-      TM::sync();
-      bitonic_sort<TM>(ws.k2, ws.v2, nP);
+      if (nhits <= 64 && ix.n_refs <= (1u << 26) && next_pow2(2 * E) <= cap.ecap) {
+        filter_masked<TM>(ws, E, nP);
+        // Sort by (reference, refPos, qPos) (Analyser.cpp:1024-1059): the
+        // packed keys order so (hit order is qPos order); then unpack.
+        bitonic_sort_keys<TM>(ws.k2, nP);
+        for (uint32_t t = tid; t < nP; t += BS) {
+          const uint64_t pk = ws.k2[t];
+          ws.k2[t] = ((pk >> 38) << 32) | ((pk >> 6) & 0xFFFFFFFFull);
+          ws.v2[t] = ws.hit_j[pk & 63];
+        }
+        TM::sync();
+      } else {
+        if (nhits < (1u << 12) && ix.n_refs < (1u << 20))
+          filter_hashed<TM>(ws, E, cap.ecap, nP);
+        else
+          filter_sorted<TM>(ws, E, nP);
+        // Sort by (reference, refPos, qPos) (Analyser.cpp:1024-1059).
+        TM::sync();
+        bitonic_sort<TM>(ws.k2, ws.v2, nP);
+      }
     }
     // Candidate references in ascending order: rank = index among the
     // distinct references; k2 becomes rank << 32 | refPos.
