@@ -2280,12 +2280,15 @@ __global__ void k_max_len(Reads rd, unsigned long long *out) {
 // then the query length.  Queries beyond the capacities go to the overflow
 // tiers.
 // References scored together by teams of 128 or more threads (and the HBM
-// tiers): each takes a 64-byte slot and a row bitmap.
+// tiers): each takes a 64-byte slot and a row bitmap of the query capacity,
+// so only workspaces of reads up to 2 kb get them (16 slots cost at most
+// 5.2 KB there; longer reads score one reference at a time).
 constexpr uint32_t kRefSlots = 16;
+constexpr uint32_t kRefSlotMaxLen = 2048;
 
 Caps caps_for(const mh_tile &t, int s, size_t limit) {
   Caps c;
-  c.nslot = t.block >= 128 ? kRefSlots : 0;
+  c.nslot = (t.block >= 128 && t.max_len <= kRefSlotMaxLen) ? kRefSlots : 0;
   const uint32_t sd = (uint32_t)(s > 0 ? s : 1);
   c.qcap = (t.max_len + 63) & ~63u;
   c.pcap = c.qcap / sd + 2;
@@ -2327,7 +2330,7 @@ size_t tile_lds(const mh_tile &t, const Caps &c) { return teams_of(t.block) * ws
 
 Caps hbm_caps(uint32_t max_read_len, int s, uint32_t ecap) {
   Caps c;
-  c.nslot = kRefSlots;
+  c.nslot = max_read_len <= kRefSlotMaxLen ? kRefSlots : 0;
   c.qcap = (max_read_len + 63) & ~63u;
   c.pcap = c.qcap / (uint32_t)(s > 0 ? s : 1) + 2;
   c.ecap = ecap;
