@@ -180,3 +180,55 @@ def test_invalid_tile_tables_are_refused(handle):
             handle.set_tiles(t)
         assert e.value.status == native.MH_ERR_ARG and "mh_set_tiles" in str(e.value), t
     assert handle.get_tiles() == saved
+
+
+WORKGROUP_TABLES = {
+    "wg256": [(200, 256, 1024, 0), (None, 256, 1024, 0)],
+    "wg128": [(200, 128, 512, 0), (None, 128, 512, 0)],
+}
+
+
+@pytest.mark.parametrize("flags", [0, native.MH_CLASSIFY_BATCHED])
+@pytest.mark.parametrize("table", sorted(WORKGROUP_TABLES))
+def test_workgroup_teams_score_many_references(handle, flags, table):
+    """Short reads in whole-workgroup teams: reads from a segment shared by 40
+    references tie across them, so a strand scores its candidates in several
+    batches of 16 reference slots, after the hit-mask filter; reads from
+    elsewhere have one candidate.  Every result and candidate equals the
+    oracle's."""
+    rng = np.random.default_rng(21)
+    seg = rng.choice(ACGT, 3000)
+    parts, off = [], [0]
+    for i in range(60):
+        r = rng.choice(ACGT, 20000)
+        if i % 3 != 2:  # 40 of the 60 references hold the segment
+            at = int(rng.integers(0, 20000 - 3000))
+            r[at:at + 3000] = seg
+        parts.append(r)
+        off.append(off[-1] + len(r))
+    rb, ro = np.concatenate(parts), np.array(off, np.uint64)
+    reads = []
+    for i in range(240):
+        if i % 2 == 0:
+            a = int(rng.integers(0, 3000 - 150))
+            reads.append(mutate(rng, seg[a:a + 150].copy(), 1))
+        else:
+            r = int(rng.integers(0, 60))
+            a = int(ro[r]) + int(rng.integers(0, 20000 - 150))
+            reads.append(mutate(rng, rb[a:a + 150].copy(), 2))
+    qb, qo = pack(reads)
+    saved = handle.get_tiles()
+    try:
+        handle.set_tiles([(ml or handle.max_read_len, b, e, g)
+                          for ml, b, e, g in WORKGROUP_TABLES[table]])
+        ix = handle.build_index(rb, ro, 30)
+        out, top, cands = ix.classify_host(qb, qo, flags=flags, want_cands=True, top_cap=64 * 240)
+        ix.free()
+    finally:
+        handle.set_tiles(saved)
+    oix = O.Index(rb, ro, 30)
+    oout, otop, ocands = oix.classify(qb, qo, want_cands=True, top_cap=64)
+    compare_reads(out, top, oout, otop, f"workgroup teams {table}")
+    compare_cands(cands, ocands, f"workgroup teams {table}")
+    per_strand = np.unique(cands[["read", "strand"]], return_counts=True)[1]
+    assert per_strand.max() > 32  # three or more batches of slots
