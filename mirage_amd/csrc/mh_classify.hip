@@ -446,6 +446,9 @@ struct Ctx {
   const uint64_t *rec;
   uint32_t rec_stride;
   uint32_t rec_maxlen;
+  // Entries the largest LDS tier holds (0: unknown).  A strand with more is
+  // passed on flagged (kBigFlag), and the LDS tiers pass it on unread.
+  uint32_t lds_ecap_max;
   Outputs out;
 };
 
@@ -455,7 +458,9 @@ constexpr uint64_t kRecAbsent = ~0ull;
 constexpr uint64_t kRecBad = ~0ull - 1;
 constexpr uint32_t kRecCntMax = 0xFFFFFFu;
 
-enum : uint32_t { kDone = 0, kOverflow = 1 };
+enum : uint32_t { kDone = 0, kOverflow = 1, kOverflowBig = 2 };
+// Overflow-list flag: the strand needs more entries than any LDS tier holds.
+constexpr uint32_t kBigFlag = 1u << 31;
 
 __device__ __forceinline__ float f32_from_bits(uint32_t u) { return __uint_as_float(u); }
 
@@ -1389,7 +1394,8 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
   if (use_rec && TM::any(big)) E = 0xFFFFFFFFu;  // a saturated count: overflow
   TM::sync();
   if (E > cap.ecap) {
-    if (!last_resort) return kOverflow;
+    if (!last_resort)
+      return (c.lds_ecap_max && E > c.lds_ecap_max && q < kBigFlag) ? kOverflowBig : kOverflow;
     res.status = MH_READ_CAPACITY;
     write_query<TM>(c, q, res);
     return kDone;
@@ -1805,6 +1811,14 @@ __device__ __forceinline__ void pass_on(const Ctx &c, uint32_t q) {
   }
 }
 
+// Handles process_query's result for q: overflowing strands go to the next
+// tier, flagged when no LDS tier can hold their entries.
+template <class TM>
+__device__ __forceinline__ void after_query(const Ctx &c, uint32_t q, uint32_t r) {
+  if (r == kOverflow) pass_on<TM>(c, q);
+  else if (r == kOverflowBig) pass_on<TM>(c, q | kBigFlag);
+}
+
 // Most queries a team takes from its queue at a time.
 constexpr uint32_t kQueueChunk = 16;
 
@@ -1835,7 +1849,7 @@ __attribute__((amdgpu_waves_per_eu(TM::kPerWave > 1 ? 4 : 1, 8))) void k_classif
   if (head == nullptr) {
     if (slot < nq) {
       const uint32_t q = qlist ? qlist[slot] : slot;
-      if (process_query<TM>(c, ws, cap, q, false, s_comp) == kOverflow) pass_on<TM>(c, q);
+      after_query<TM>(c, q, process_query<TM>(c, ws, cap, q, false, s_comp));
     }
     stats_flush<TM>(ws, c.out.ctr, slot);
     return;
@@ -1858,7 +1872,11 @@ __attribute__((amdgpu_waves_per_eu(TM::kPerWave > 1 ? 4 : 1, 8))) void k_classif
     const uint32_t end = (uint32_t)min((unsigned long long)base + chunk, n);
     for (uint32_t i = base + team; i < end; i += G) {
       const uint32_t q = qlist ? qlist[i] : i;
-      if (process_query<TM>(c, ws, cap, q, false, s_comp) == kOverflow) pass_on<TM>(c, q);
+      if (q & kBigFlag) {  // more entries than an LDS tier holds: straight on
+        pass_on<TM>(c, q);
+        continue;
+      }
+      after_query<TM>(c, q, process_query<TM>(c, ws, cap, q, false, s_comp));
     }
   }
   stats_flush<TM>(ws, c.out.ctr, slot);
@@ -1889,8 +1907,8 @@ __global__ __launch_bounds__(BS) void k_classify_hbm(Ctx c, Caps cap, unsigned c
     }
     __syncthreads();
     if (slot == 0xFFFFFFFFu) break;
-    const uint32_t q = list[slot];
-    if (process_query<TM>(c, ws, cap, q, last_resort != 0, s_comp) == kOverflow)
+    const uint32_t q = list[slot] & ~kBigFlag;
+    if (process_query<TM>(c, ws, cap, q, last_resort != 0, s_comp) != kDone)
       pass_on<TM>(c, q);
   }
   stats_flush<TM>(ws, c.out.ctr, blockIdx.x);
@@ -2451,6 +2469,7 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
   c.rec = nullptr;
   c.rec_stride = 0;
   c.rec_maxlen = 0;
+  c.lds_ecap_max = 0;
   c.out = a.out;
   c.out.q = qo;
   c.out.ctr = x.d_ctr;
@@ -2494,6 +2513,9 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
         x.timed_buckets = 1;
       }
     } else {
+      // entries of the middle tier's workspace (the largest LDS tier below)
+      c.lds_ecap_max =
+          tile_caps(mh_tile{h->max_read_len, 256, 4096, 0}, a.ix.s, h->lds_per_block).ecap;
       const uint32_t nb = (uint32_t)std::min<size_t>(tiles.size(), 8);
       Lens d_maxlen{};
       for (uint32_t i = 0; i < nb; i++) d_maxlen.v[i] = tiles[i].max_len;
@@ -2554,7 +2576,7 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
     DevCounters *d = x.d_ctr;
     const mh_tile lds_tiers[2] = {
         {std::min<uint32_t>(1024u, h->max_read_len), 128, 512, 0},
-        {h->max_read_len, 256, 4096, 0}};
+        {h->max_read_len, 256, 4096, 0}};  // lds_ecap_max is this tier's capacity
     for (int t = 0; t < 2; t++) {
       const Caps tc = tile_caps(lds_tiers[t], a.ix.s, h->lds_per_block);
       Ctx ct = c;
