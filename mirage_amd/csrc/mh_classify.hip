@@ -2328,8 +2328,11 @@ Caps caps_for(const mh_tile &t, int s, size_t limit) {
 
 // HBM tiers: a first one with moderate workspaces over many workgroups,
 // then the last resort with the largest entry capacity over a few.  They run
-// one after the other and share one region.
-constexpr uint32_t kHbmBlocks = 512;
+// one after the other and share one region.  The first tier's strands wait on
+// global memory, so it runs as many 256-thread workgroups as are resident at
+// once (four per CU at its register count): 1.3 GiB of workspaces for 32 kb
+// reads.
+constexpr uint32_t kHbmBlocks = 1024;
 constexpr uint32_t kHbmEcap = 1u << 15;
 constexpr uint32_t kBigBlocks = 32;
 constexpr uint32_t kBigEcap = 1u << 19;
