@@ -96,6 +96,19 @@ def test_parse_unaligned_and_resized(handle):
     assert exact[1].numel() == 5001
 
 
+def test_parse_long_unwrapped_lines(handle):
+    """Sequence lines of 1.5 Mb and 300 kb (unwrapped chromosomes): a line
+    spans hundreds of wavefront windows; CRLF and lowercase included."""
+    rng = np.random.default_rng(17)
+    s1 = rng.choice(np.frombuffer(b"ACGTacgtN", np.uint8), 1_500_000).tobytes()
+    s2 = rng.choice(np.frombuffer(b"ACGT", np.uint8), 300_000).tobytes()
+    text = b">chr1 long\r\n" + s1 + b"\r\n>chr2\n" + s2[:150_000] + b"\n" + s2[150_000:]
+    seqs, heads = device_parse(handle, text)
+    exp = F.parse(text)
+    assert heads == exp[1] and [len(x) for x in seqs] == [len(x) for x in exp[0]]
+    assert seqs == exp[0]
+
+
 @pytest.mark.parametrize("text,what", [
     (b">a\nACGT\n\nACGT\n", "empty line"),
     (b">a\nACGT\n@b\nACGT\n", "'@'"),
