@@ -351,6 +351,72 @@ __device__ void bitonic_sort_keys(uint64_t *key, uint32_t n) {
   TM::sync();
 }
 
+// Ascending sort of n u64 keys in global memory through an LDS buffer of
+// lcap keys (a power of two), for workspaces that live in HBM.  Same network
+// as bitonic_sort; every run of its steps whose stride is below the chunk
+// size m = min(next_pow2(n), lcap) is done one m-key chunk at a time in LDS
+// (a chunk's tail past n padded with ~0), the larger strides in global
+// memory, where positions n and beyond act as +infinity.  For n2 = 2^17 and
+// lcap = 2^12 that is 15 global steps and 6 LDS passes in place of 153
+// global steps.  Keys need not be distinct.
+template <class TM>
+__device__ void staged_sort_keys(uint64_t *key, uint32_t n, uint64_t *lds, uint32_t lcap) {
+  if (n <= 1) {
+    TM::sync();
+    return;
+  }
+  const uint32_t n2 = next_pow2(n);
+  const uint32_t m = min(n2, lcap);
+  auto step = [&](uint64_t *a, uint32_t len, uint32_t size, uint32_t stride, bool mirror) {
+    const uint32_t lg = 31 - __clz(stride);
+    for (uint32_t t = TM::rank(); t < (len >> 1); t += TM::kSize) {
+      const uint32_t i = ((t >> lg) << (lg + 1)) + (t & (stride - 1));
+      const uint32_t j = mirror ? (i ^ (size - 1)) : (i + stride);
+      if (j >= n) continue;
+      const uint64_t ka = a[i], kb = a[j];
+      if (kb < ka) {
+        a[i] = kb;
+        a[j] = ka;
+      }
+    }
+  };
+  // full: sizes 2..m; else the strides m/2..1 of a merge already past them
+  auto chunks = [&](bool full) {
+    for (uint32_t c0 = 0; c0 < n; c0 += m) {
+      TM::sync();
+      for (uint32_t i = TM::rank(); i < m; i += TM::kSize)
+        lds[i] = c0 + i < n ? key[c0 + i] : ~0ull;
+      for (uint32_t size = full ? 2 : m; size <= m; size <<= 1)
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+          TM::sync();
+          // inside the chunk every position is loaded (pads are ~0)
+          const uint32_t lg = 31 - __clz(stride);
+          const bool mirror = full && stride == (size >> 1);
+          for (uint32_t t = TM::rank(); t < (m >> 1); t += TM::kSize) {
+            const uint32_t i = ((t >> lg) << (lg + 1)) + (t & (stride - 1));
+            const uint32_t j = mirror ? (i ^ (size - 1)) : (i + stride);
+            const uint64_t ka = lds[i], kb = lds[j];
+            if (kb < ka) {
+              lds[i] = kb;
+              lds[j] = ka;
+            }
+          }
+        }
+      TM::sync();
+      for (uint32_t i = TM::rank(); i < m && c0 + i < n; i += TM::kSize) key[c0 + i] = lds[i];
+    }
+    TM::sync();
+  };
+  chunks(true);
+  for (uint32_t size = m << 1; size <= n2; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride >= m; stride >>= 1) {
+      step(key, n2, size, stride, stride == (size >> 1));
+      TM::sync();
+    }
+    chunks(false);
+  }
+}
+
 // Workspace carved from LDS (or HBM for the large kernel).
 struct Ws {
   uint8_t *q;          // qcap query bases
@@ -368,6 +434,8 @@ struct Ws {
   unsigned long long *stats;  // kNumStats, per workgroup
   struct RefSlot *slots;  // nslot references scored together
   uint32_t *slot_bm;      // nslot row bitmaps of qcap / 32 + 1 words
+  uint64_t *stage;        // LDS sort buffer of stage_cap keys, or null
+  uint32_t stage_cap;     // a power of two
 };
 
 // One reference of a multi-reference scoring batch.
@@ -430,6 +498,8 @@ __device__ inline Ws ws_carve(unsigned char *base, const Caps &c) {
   w.stats = (unsigned long long *)take(64);
   w.slots = (RefSlot *)take(c.nslot * sizeof(RefSlot));
   w.slot_bm = (uint32_t *)take(c.nslot * (c.qcap / 32 + 1) * 4ull);
+  w.stage = nullptr;
+  w.stage_cap = 0;
   return w;
 }
 
@@ -896,7 +966,10 @@ __device__ void filter_hashed(const Ws &ws, uint32_t E, uint32_t ecap, uint32_t 
   TM::sync();
   uint32_t maxrun = 0;
   if (nm) {
-    bitonic_sort<TM>(ws.k2, ws.t3, nm);  // t3: value scratch, unused
+    if (ws.stage)
+      staged_sort_keys<TM>(ws.k2, nm, ws.stage, ws.stage_cap);
+    else
+      bitonic_sort<TM>(ws.k2, ws.t3, nm);  // t3: value scratch, unused
     maxrun = run_lengths<TM, 32, 12>(ws, ws.k2, nm, ws.t3, ws.rref);
     for (uint32_t t = tid; t < nm; t += BS) atomicMax(&rval[(uint32_t)ws.k2[t]], ws.t3[t]);
     TM::sync();
@@ -912,15 +985,20 @@ __device__ void filter_hashed(const Ws &ws, uint32_t E, uint32_t ecap, uint32_t 
     if (e < E) {
       key = ws.k1[e];
       pos = ws.v1[e];
-      j = ws.hit_j[(uint32_t)key];
+      if (!ws.stage) j = ws.hit_j[(uint32_t)key];
       keep = rval[ws.v2[e]] >= cut ? 1u : 0u;
     }
     uint32_t tot;
     const uint32_t ex = TM::excl_scan(keep, ws.red, tot);
     TM::sync();
     if (keep) {
-      ws.k2[nkeep + ex] = (key & 0xFFFFFFFF00000000ull) | pos;
-      ws.v2[nkeep + ex] = j;
+      if (ws.stage) {
+        // packed for a key-only sort: ref << 44 | refPos << 12 | hit
+        ws.k2[nkeep + ex] = ((key >> 32) << 44) | ((uint64_t)pos << 12) | (uint32_t)key;
+      } else {
+        ws.k2[nkeep + ex] = (key & 0xFFFFFFFF00000000ull) | pos;
+        ws.v2[nkeep + ex] = j;
+      }
     }
     nkeep += tot;
   }
@@ -1553,7 +1631,10 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
         filter_masked<TM>(ws, E, nP);
         // Sort by (reference, refPos, qPos) (Analyser.cpp:1024-1059): the
         // packed keys order so (hit order is qPos order); then unpack.
-        bitonic_sort_keys<TM>(ws.k2, nP);
+        if (ws.stage)
+          staged_sort_keys<TM>(ws.k2, nP, ws.stage, ws.stage_cap);
+        else
+          bitonic_sort_keys<TM>(ws.k2, nP);
         for (uint32_t t = tid; t < nP; t += BS) {
           const uint64_t pk = ws.k2[t];
           ws.k2[t] = ((pk >> 38) << 32) | ((pk >> 6) & 0xFFFFFFFFull);
@@ -1561,13 +1642,25 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
         }
         TM::sync();
       } else {
-        if (nhits < (1u << 12) && ix.n_refs < (1u << 20))
+        const bool hashed = nhits < (1u << 12) && ix.n_refs < (1u << 20);
+        if (hashed)
           filter_hashed<TM>(ws, E, cap.ecap, nP);
         else
           filter_sorted<TM>(ws, E, nP);
         // Sort by (reference, refPos, qPos) (Analyser.cpp:1024-1059).
         TM::sync();
-        bitonic_sort<TM>(ws.k2, ws.v2, nP);
+        if (hashed && ws.stage) {
+          // filter_hashed packed the pairs as ref << 44 | refPos << 12 | hit
+          staged_sort_keys<TM>(ws.k2, nP, ws.stage, ws.stage_cap);
+          for (uint32_t t = tid; t < nP; t += BS) {
+            const uint64_t pk = ws.k2[t];
+            ws.k2[t] = ((pk >> 44) << 32) | ((pk >> 12) & 0xFFFFFFFFull);
+            ws.v2[t] = ws.hit_j[pk & 0xFFF];
+          }
+          TM::sync();
+        } else {
+          bitonic_sort<TM>(ws.k2, ws.v2, nP);
+        }
       }
     }
     // Candidate references in ascending order: rank = index among the
@@ -1886,13 +1979,20 @@ __attribute__((amdgpu_waves_per_eu(TM::kPerWave > 1 ? 4 : 1, 8))) void k_classif
 // workspace per workgroup, queries taken one at a time from `list`.  The
 // last resort flags what it cannot hold; the tier before it passes such
 // queries on.
+constexpr uint32_t kStageKeys = 4096;
+
 template <int BS>
-__global__ __launch_bounds__(BS) void k_classify_hbm(Ctx c, Caps cap, unsigned char *scratch,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_classify_hbm(Ctx c, Caps cap, unsigned char *scratch,
                                                      size_t ws_stride, const uint32_t *list,
                                                      const unsigned long long *count,
                                                      unsigned long long *head, int last_resort) {
   using TM = BlockTeam<BS>;
-  const Ws ws = ws_carve(scratch + blockIdx.x * ws_stride, cap);
+  // 32 KiB of LDS per workgroup for the sorts: four resident 256-thread
+  // workgroups per CU fit in its 160 KiB
+  __shared__ uint64_t s_stage[kStageKeys];
+  Ws ws = ws_carve(scratch + blockIdx.x * ws_stride, cap);
+  ws.stage = s_stage;
+  ws.stage_cap = kStageKeys;
   __shared__ uint32_t slot;
   __shared__ uint8_t s_comp[256];
   for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_comp[i] = kComp[i];
