@@ -109,9 +109,10 @@ typedef struct {
   uint64_t seeds;          /* seeds scored */
   uint64_t cands;          /* seeded candidates */
   uint64_t direct;         /* strands resolved by the direct-hit fast path */
-  uint64_t overflow;       /* tier escalations: strands passed from their length tile
-                              to the middle tier, from it to the HBM tiers, and from
-                              the first HBM tier to the last resort */
+  uint64_t overflow;       /* tier escalations: each pass of a strand from its length
+                              tile to the overflow tiers, and from one overflow tier to
+                              the next (second chance, middle, HBM workspaces, last
+                              resort) */
   uint64_t ext_compares;   /* base comparisons of head/tail extension */
 } mh_counters;
 
@@ -119,7 +120,7 @@ typedef struct {
 typedef struct {
   float total_ms;          /* first to last kernel of the call */
   float bucket_ms[8];      /* LDS kernel per length bucket (plain form: [0]) */
-  float big_ms;            /* middle tier and HBM-workspace tiers */
+  float big_ms;            /* overflow tiers: second chance, middle, HBM workspaces */
   float merge_ms;          /* strand merge */
   uint32_t n_buckets;
   float probe_ms;          /* probe pre-pass (MH_CLASSIFY_PREPROBE), else 0 */
