@@ -1746,7 +1746,7 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
   uint32_t seeded = 0;
   bool multi = false;
   if constexpr (TM::kPerWave == 1) {
-    if (cap.nslot != 0) {
+    if (cap.nslot != 0 && nranks >= 2) {
       multi = true;
       uint32_t lut_bad = 0;
       score_multi<TM>(c, ws, cap, m, nranks, nchains, G, nhits, nph, ultrafast, read, strand,
@@ -2398,10 +2398,12 @@ __global__ void k_max_len(Reads rd, unsigned long long *out) {
 // first the entry capacity (to what fits, multiples of 16, down to 256),
 // then the query length.  Queries beyond the capacities go to the overflow
 // tiers.
-// References scored together by teams of 128 or more threads (and the HBM
-// tiers): each takes a 64-byte slot and a row bitmap of the query capacity,
-// so only workspaces of reads up to 2 kb get them (16 slots cost at most
-// 5.2 KB there; longer reads score one reference at a time).
+// References scored together by teams of 128 or more threads and by the HBM
+// tiers, for strands with two or more candidate references: each takes a
+// 64-byte slot and a row bitmap of the query capacity.  In LDS only
+// workspaces of reads up to 2 kb get them (16 slots cost at most 5.2 KB
+// there; longer reads score one reference at a time); HBM workspaces get
+// them at any length (about 66 KB each for 32 kb reads).
 constexpr uint32_t kRefSlots = 16;
 constexpr uint32_t kRefSlotMaxLen = 2048;
 
@@ -2452,7 +2454,7 @@ size_t tile_lds(const mh_tile &t, const Caps &c) { return teams_of(t.block) * ws
 
 Caps hbm_caps(uint32_t max_read_len, int s, uint32_t ecap) {
   Caps c;
-  c.nslot = max_read_len <= kRefSlotMaxLen ? kRefSlots : 0;
+  c.nslot = kRefSlots;
   c.qcap = (max_read_len + 63) & ~63u;
   c.pcap = c.qcap / (uint32_t)(s > 0 ? s : 1) + 2;
   c.ecap = ecap;
