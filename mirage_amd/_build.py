@@ -24,7 +24,7 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off", "-fno-
 
 HIP_SOURCES = ["mh_index.hip", "mh_classify.hip", "mh_api.hip", "mh_serve.hip",
                "mh_autotune.hip", "mh_mhm.hip", "mh_fasta.hip"]
-HOST_SOURCES = ["matrix_hip.cpp", "mh_results.cpp"]
+HOST_SOURCES = ["matrix_hip.cpp", "matrix_hip_check.cpp", "mh_results.cpp"]
 
 
 def datagen_lib_path():

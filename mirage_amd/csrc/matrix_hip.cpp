@@ -62,9 +62,11 @@ void Device::LoadReferences(const std::vector<std::vector<char>> &refs, int32_t 
     if (!refs[i].empty()) std::memcpy(bytes.data() + off[i], refs[i].data(), refs[i].size());
   if (ix_) mh_index_free(ix_);
   ix_ = nullptr;
+  refLens_.clear();
   check(h_, mh_index_build(h_, bytes.data(), off.data(), (uint32_t)refs.size(), T,
                            MH_INDEX_LAST_BUCKET_EMPTY, &ix_),
         "mh_index_build");
+  for (const auto &r : refs) refLens_.push_back(r.size());
 }
 
 size_t Device::MaxReadLength() const { return mh_max_read_len(h_); }
@@ -78,6 +80,16 @@ std::vector<MatrixResult> Device::GetHighestScore(
     const std::vector<std::vector<std::vector<Position>>> &cpl, bool ultraFast) {
   if (!ix_) throw Error(MH_ERR_STATE, "no reference index loaded");
   const size_t nq = seqQuery.size();
+  {
+    std::vector<size_t> qlen(nq);
+    for (size_t i = 0; i < nq; i++) {
+      if (!seqQuery[i])
+        throw Error(MH_ERR_ARG, "GetHighestScore: query " + std::to_string(i) + " is NULL");
+      qlen[i] = seqQuery[i]->size();
+    }
+    const std::string bad = CheckScoringInputs(qlen, refLens_, MaxReadLength(), qrc, cp, cpl);
+    if (!bad.empty()) throw Error(MH_ERR_ARG, "GetHighestScore: " + bad);
+  }
   std::vector<uint64_t> qoff(nq + 1, 0);
   for (size_t i = 0; i < nq; i++) qoff[i + 1] = qoff[i] + seqQuery[i]->size();
   std::vector<uint8_t> qbytes(qoff.back());

@@ -40,6 +40,21 @@ struct MatrixResult {
   static float GetRelativeScore(float querySize, float referenceSize, float score);
 };
 
+// Checks the scoring inputs of Device::GetHighestScore on the host before
+// anything is uploaded: one candidate list and one seed list per query, as
+// many last positions as start positions, every candidate a reference id
+// below refLens.size(), and every seed that is scored (start.Query <=
+// last.Query; MatrixCPU.cpp:57-58 skips the others) inside its query
+// (last.Query <= query length) and its reference (start.Reference < length,
+// last.Reference <= length).  Queries longer than maxQueryLen are refused
+// too.  Returns "" when the inputs are well formed, else a message naming the
+// query, candidate and seed.  Host only; it touches no device.
+std::string CheckScoringInputs(const std::vector<size_t> &queryLens,
+                               const std::vector<uint64_t> &refLens, size_t maxQueryLen,
+                               const std::vector<std::vector<SEQ_ID_t>> &queryReferenceCombinations,
+                               const std::vector<std::vector<std::vector<Position>>> &combiPositions,
+                               const std::vector<std::vector<std::vector<Position>>> &combiPositionsLast);
+
 class Error : public std::runtime_error {
  public:
   Error(mh_status st, const std::string &what) : std::runtime_error(what), status(st) {}
@@ -91,6 +106,7 @@ class Device {
  private:
   mh_handle *h_ = nullptr;
   mh_index *ix_ = nullptr;
+  std::vector<uint64_t> refLens_;  // lengths of the indexed references
 };
 
 }  // namespace mirage_hip

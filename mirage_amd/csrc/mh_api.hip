@@ -274,7 +274,9 @@ static mh_status classify_on(mh_handle *h, Exec &x, const mh_index *idx, const u
     // otherwise capture it again.  The scratch arena is sized before the
     // capture (no allocation or synchronisation may happen inside it).
     a.flags = (flags | MH_CLASSIFY_BATCHED) & ~(MH_CLASSIFY_TIMING | MH_CLASSIFY_GRAPH);
-    const uint64_t key[12] = {(uint64_t)idx, (uint64_t)d_reads, (uint64_t)d_read_off, n_reads,
+    // The index enters by its generation: a new index may reuse a freed
+    // one's address, and the captured kernels hold the old one's buffers.
+    const uint64_t key[12] = {idx->generation, (uint64_t)d_reads, (uint64_t)d_read_off, n_reads,
                               a.flags, (uint64_t)d_out, (uint64_t)d_top_pool, top_cap,
                               (uint64_t)d_cands, cand_cap, tiles_version(h), x.tie_factor};
     if (!x.graph || std::memcmp(key, x.graph_key, sizeof(key)) != 0) {

@@ -1421,6 +1421,9 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
 
   // Probe every s-th window; compact present hashes into the hit list.
   // With probe records, the lookups were made beforehand by k_probe.
+  // elim: the entry count past which the strand leaves this workspace
+  // (the LDS tiers' largest counts too, which route an overflowing strand).
+  const uint32_t elim = max(cap.ecap, c.lds_ecap_max);
   uint32_t nhits = 0, big = 0, ph = 0;
   for (uint32_t base = 0; base < nprobe; base += BS) {
     const uint32_t p = base + tid;
@@ -1443,7 +1446,9 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
           lo = ix.dir[h];
           const uint64_t hi = ix.dir[h + 1];
           present = hi > lo;
-          cnt = (uint32_t)(hi - lo);
+          // A bucket larger than every LDS workspace overflows on its own:
+          // clamp, so that the 32-bit entry sums below cannot wrap.
+          cnt = (uint32_t)min(hi - lo, (uint64_t)elim + 1);
         }
       }
     }
@@ -1460,16 +1465,24 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
   // Probes that hit bucket 0 while it holds unstored copies of (0, 0).
   const uint32_t nph = ix.phantom ? TM::sum(ph, ws.red) : 0u;
   TM::sync();
+  // Entry total E and the hit_base prefix sums.  Every count is at most
+  // elim + 1 <= 2^19 + 1 (probe records saturate at kRecCntMax), so one round
+  // adds less than 2^30 to an E that is at most elim: once E exceeds elim
+  // the strand overflows (E saturates) and the sums stop before they wrap.
   uint32_t E = 0;
   for (uint32_t base = 0; base < nhits; base += BS) {
     const uint32_t i = base + tid;
-    const uint32_t cnt = i < nhits ? ws.hit_base[i] : 0;
+    const uint32_t cnt = i < nhits ? min(ws.hit_base[i], elim + 1) : 0;
     uint32_t tot;
     const uint32_t ex = TM::excl_scan(cnt, ws.red, tot);
     if (i < nhits) ws.hit_base[i] = E + ex;
     E += tot;
+    if (E > elim) {
+      E = 0xFFFFFFFFu;
+      break;
+    }
   }
-  if (tid == 0) ws.hit_base[nhits] = E;
+  if (tid == 0 && E <= cap.ecap) ws.hit_base[nhits] = E;
   if (use_rec && TM::any(big)) E = 0xFFFFFFFFu;  // a saturated count: overflow
   TM::sync();
   if (E > cap.ecap) {
@@ -2311,6 +2324,51 @@ __global__ __launch_bounds__(64) void k_score_seeds(IndexView ix, Luts lut, cons
   }
 }
 
+// Seed check of mh_score_seeds, before any scoring: the scoring kernel
+// trusts every seed it extends.  A seed (start, last) with start.query <=
+// last.query is scored (MatrixCPU.cpp:57-58 skips the others), so it must
+// lie inside its query (last.query <= m: the row bitmap holds m bits) and
+// its reference (start.ref < ref_len: the head extension reads the
+// reference byte at start.ref; last.ref <= ref_len).  The first offending
+// pair is kept as pair << 8 | code (atomicMin: the lowest pair wins).
+enum SeedCheck : uint32_t {
+  kSeedOk = 0, kSeedRefId = 1, kSeedQueryLen = 2, kSeedOffsets = 3, kSeedNull = 4,
+  kSeedQuery = 5, kSeedRef = 6
+};
+__global__ __launch_bounds__(256) void k_check_seeds(uint32_t n_refs, const uint64_t *ref_off,
+                                                     const uint64_t *qoff, const uint32_t *pq,
+                                                     const uint32_t *pr, const uint64_t *soff,
+                                                     const uint32_t *seeds, uint32_t n_pairs,
+                                                     uint32_t max_len,
+                                                     unsigned long long *first_bad) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs;
+       p += gridDim.x * blockDim.x) {
+    uint32_t code = kSeedOk;
+    const uint32_t qi = pq[p], R = pr[p];
+    const uint64_t m = qoff[qi + 1] - qoff[qi];
+    const uint64_t s0 = soff[p], s1 = soff[p + 1];
+    if (R >= n_refs) code = kSeedRefId;
+    else if (m > max_len) code = kSeedQueryLen;
+    else if (s1 < s0) code = kSeedOffsets;
+    else if (s1 > s0 && seeds == nullptr) code = kSeedNull;
+    else {
+      const uint64_t ref_len = ref_off[R + 1] - ref_off[R];
+      for (uint64_t i = s0; i < s1 && code == kSeedOk; i++) {
+        const uint32_t *sd = seeds + 4 * i;  // start.ref, start.query, last.ref, last.query
+        const uint32_t sref = sd[0], sq = sd[1], lref = sd[2], lq = sd[3];
+        if (sq > lq) continue;
+        if (lq > m) code = kSeedQuery;
+        else if (sref >= ref_len || lref > ref_len) code = kSeedRef;
+      }
+    }
+    if (code != kSeedOk) atomicMin(first_bad, ((unsigned long long)p << 8) | code);
+  }
+}
+
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
 bool g_comp_ready[64] = {};
 
 }  // namespace
@@ -2756,6 +2814,33 @@ mh_status launch_score_seeds(mh_handle *h, const IndexView &ix, const Luts &luts
                              uint32_t n_pairs, uint32_t flags, float *d_score, float *d_rel,
                              uint64_t *d_sum_r, uint64_t *d_sum_c, uint32_t *d_hits) {
   if (!n_pairs) return MH_OK;
+  {
+    // Every pair and seed is checked before anything is scored.
+    Exec &x = h->ex;
+    unsigned long long *bad = &x.d_ctr->seed_check;
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, x.stream, bad, ~0ull);
+    const uint32_t cgrid = std::min<uint32_t>((n_pairs + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_check_seeds, dim3(cgrid), dim3(256), 0, x.stream, ix.n_refs,
+                       ix.ref_off, d_qoff, d_pq, d_pr, d_soff, d_seeds, n_pairs,
+                       h->max_read_len, bad);
+    MH_HIP(h, hipGetLastError());
+    MH_HIP(h, hipMemcpyAsync(&x.h_ctr->seed_check, bad, 8, hipMemcpyDeviceToHost, x.stream));
+    MH_HIP(h, hipStreamSynchronize(x.stream));
+    const unsigned long long v = x.h_ctr->seed_check;
+    if (v != ~0ull) {
+      static const char *what[] = {
+          "", "reference id not below the index's reference count",
+          "query longer than the handle's maximum read length",
+          "seed offsets decrease", "seeds are NULL but the pair has seeds",
+          "a seed ends past its query (last.query > query length)",
+          "a seed lies outside its reference (start.ref >= length or last.ref > length)"};
+      const uint32_t code = (uint32_t)(v & 0xFF);
+      const uint64_t pair = v >> 8;
+      return set_error(h, code == kSeedQueryLen ? MH_ERR_RANGE : MH_ERR_ARG,
+                       "mh_score_seeds: pair " + std::to_string(pair) + ": " +
+                           (code < 7 ? what[code] : "bad pair"));
+    }
+  }
   const uint32_t qcap = (h->max_read_len + 63) & ~63u;
   const size_t lds = al8(qcap) + al8((qcap / 32 + 1) * 4ull) + 64 * 4 + 16;
   const uint32_t grid = std::min<uint32_t>(n_pairs, 65536);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -74,7 +75,7 @@ struct DevCounters {
   unsigned long long top_used;      // final top-tie pool fill (set by the merge)
   unsigned long long cand_used;
   unsigned long long tie_used;      // scratch tie pool fill
-  unsigned long long pad;
+  unsigned long long seed_check;    // first bad pair of mh_score_seeds (pair << 8 | code)
   unsigned long long tier_n[4];     // queries passed on to overflow tier t
   unsigned long long tier_head[4];  // queue heads of the overflow tiers
   unsigned long long queue_head[8]; // queue heads of the length buckets
@@ -102,9 +103,17 @@ struct Caps {
   uint32_t nslot = 0;  // references scored together (workgroup teams); 0: one at a time
 };
 
+
+// Identity of an index for the lifetime of the process: a freed index's
+// address can be reused by the next one, its generation never is.
+inline uint64_t next_index_generation() {
+  static std::atomic<uint64_t> g{0};
+  return ++g;
+}
 }  // namespace mh
 
 struct mh_index {
+  const uint64_t generation = mh::next_index_generation();
   mh_handle *owner = nullptr;
   int device = 0;
   int32_t threshold = 0;

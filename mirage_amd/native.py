@@ -401,6 +401,33 @@ class DeviceIndex:
             return out, top[:tn.value], cands
         raise MirageError(MH_ERR_OVERFLOW, "output pools kept overflowing")
 
+    def score_seeds(self, queries, query_off, pair_query, pair_ref, seed_off, seeds, flags=0):
+        """Scoring only (mh_score_seeds) from host arrays: pair p scores query
+        pair_query[p] against reference pair_ref[p] with seeds
+        seeds[seed_off[p]:seed_off[p+1]] as (start.ref, start.query, last.ref,
+        last.query) rows.  Returns (score, rel, sum_r, sum_c, hits) arrays."""
+        import torch
+        q = np.ascontiguousarray(queries, dtype=np.uint8)
+        qo = np.ascontiguousarray(query_off, dtype=np.uint64)
+        pq = np.ascontiguousarray(pair_query, dtype=np.uint32)
+        pr = np.ascontiguousarray(pair_ref, dtype=np.uint32)
+        so = np.ascontiguousarray(seed_off, dtype=np.uint64)
+        sd = np.ascontiguousarray(seeds, dtype=np.uint32).reshape(-1)
+        n = len(pq)
+        dev = lambda a: torch.from_numpy(a.view(np.uint8).copy() if a.size else  # noqa: E731
+                                         np.zeros(8, np.uint8)).cuda()
+        d_q, d_qo, d_pq, d_pr, d_so = dev(q), dev(qo), dev(pq), dev(pr), dev(so)
+        d_sd = dev(sd) if sd.size else None
+        outs = [torch.zeros(max(n, 1) * w, dtype=torch.uint8, device="cuda") for w in (4, 4, 8, 8, 4)]
+        _torch_sync(d_q)
+        st = self.handle.lib.mh_score_seeds(
+            self.handle.h, self.ix, _dev_ptr(d_q), _dev_ptr(d_qo), _dev_ptr(d_pq), _dev_ptr(d_pr),
+            _dev_ptr(d_so), _dev_ptr(d_sd), n, flags, *[_dev_ptr(o) for o in outs])
+        self.handle.check(st, "mh_score_seeds")
+        torch.cuda.synchronize()
+        types = (np.float32, np.float32, np.uint64, np.uint64, np.uint32)
+        return tuple(o.cpu().numpy().view(t)[:n] for o, t in zip(outs, types))
+
     def classify_device(self, d_reads, d_read_off, n_reads, d_out, d_top, top_cap, flags=0,
                         d_cands=None, cand_cap=0, wait=True, sync_inputs=True):
         """Full path on device-resident torch tensors (mh_classify).  The

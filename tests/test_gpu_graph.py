@@ -91,3 +91,42 @@ def test_graph_captured_again_after_arena_grows():
     assert run(graph, n) == eager_all
     ix.free()
     h.close()
+
+
+def test_graph_not_replayed_for_a_new_index_at_a_reused_address():
+    """A captured call holds the index's buffers.  Freeing the index and
+    building another (whose handle may come back at the same host address)
+    must capture again: the graph call equals the eager call on the new
+    index, not the replay of kernels reading the freed one."""
+    import torch
+    h = native.Handle(0, 4096)
+    graph = native.MH_CLASSIFY_BATCHED | native.MH_CLASSIFY_GRAPH
+    results = []
+    w = D.workload(1, 0.05)
+    rb, ro, qb, qo, _ = D.generate(w)
+    n = len(qo) - 1
+    d_reads = torch.from_numpy(qb).cuda()
+    d_off = torch.from_numpy(qo.view(np.int64)).cuda()
+    cap = 16 * n + 64
+    d_out = torch.zeros(n * native.READ_RESULT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    d_top = torch.zeros(cap * 4, dtype=torch.uint8, device="cuda")
+    for variant in range(3):
+        # a different reference set each time: the reads match only the first
+        rb2 = rb.copy()
+        if variant:
+            rng = np.random.default_rng(variant)
+            rb2[:] = np.frombuffer(b"ACGT", np.uint8)[rng.integers(0, 4, len(rb2))]
+        ix = h.build_index(rb2, ro, w.threshold)
+        per = []
+        for flags in (native.MH_CLASSIFY_BATCHED, graph):
+            d_out.fill_(0xAB)
+            torch.cuda.synchronize()
+            ix.classify_device(d_reads, d_off, n, d_out, d_top, cap, flags=flags)
+            out = d_out.cpu().numpy().view(native.READ_RESULT_DTYPE)
+            top = d_top.cpu().numpy().view(np.uint32)
+            per.append((out["highest"].view(np.uint32).tolist(), native.tie_sets(out, top)))
+        assert per[0] == per[1], f"variant {variant}: graph call differs from eager"
+        results.append(per[0])
+        ix.free()
+    assert results[0] != results[1]
+    h.close()

@@ -75,3 +75,52 @@ def test_shim_classify_and_score_seeds():
         assert int(f[1], 16) == fbits(best)
         got = [(int(f[3 + 2 * k], 16), int(f[4 + 2 * k])) for k in range(int(f[2]))]
         assert got == scores
+
+
+def test_score_seeds_checks_every_seed(handle):
+    """mh_score_seeds refuses malformed pairs before scoring anything
+    (MH_ERR_ARG, or MH_ERR_RANGE for a query longer than the handle's
+    maximum), naming the first bad pair; well-formed seeds score as the
+    oracle does, and seeds with start.query > last.query are skipped
+    (MatrixCPU.cpp:57-58)."""
+    from mirage_amd import native
+    rng = np.random.default_rng(11)
+    refs = [bytes(rng.choice(list(b"ACGT"), n).astype(np.uint8)) for n in (400, 300)]
+    rb, ro = _data.pack([r.decode() for r in refs])
+    T = 30
+    ix = handle.build_index(rb, ro, T)
+    q = bytearray(rng.choice(list(b"ACGT"), 120).astype(np.uint8))
+    q[10:70] = refs[0][100:160]
+    queries = np.frombuffer(bytes(q), np.uint8)
+    qoff = np.array([0, len(q)], np.uint64)
+    good = [(100, 10, 160, 70)]
+    skipped = [(390, 50, 300, 40)]  # start.query > last.query: skipped, positions never read
+
+    def call(pair_ref, seeds_per_pair, q_=queries, qo_=qoff):
+        so = np.zeros(len(pair_ref) + 1, np.uint64)
+        so[1:] = np.cumsum([len(x) for x in seeds_per_pair])
+        flat = np.array([v for x in seeds_per_pair for sd in x for v in sd], np.uint32)
+        return ix.score_seeds(q_, qo_, np.zeros(len(pair_ref), np.uint32), pair_ref, so, flat)
+
+    score, rel, sr, sc, hits = call([0, 0], [good, good + skipped])
+    exp = O.score_seeds(bytes(q), refs[0], good, T)
+    for p in range(2):
+        assert np.float32(score[p]).view(np.uint32) == np.float32(exp[0]).view(np.uint32)
+        assert (int(sr[p]), int(sc[p]), int(hits[p])) == tuple(exp[1:])
+    bad_cases = [
+        ([0, 2], [good, good], "pair 1", "reference id"),
+        ([0, 0, 1], [good, good, [(100, 10, 160, 121)]], "pair 2", "ends past its query"),
+        ([1], [[(300, 10, 310, 20)]], "pair 0", "outside its reference"),
+        ([1], [[(290, 10, 301, 20)]], "pair 0", "outside its reference"),
+    ]
+    for pair_ref, seeds, where, what in bad_cases:
+        with pytest.raises(native.MirageError) as e:
+            call(pair_ref, seeds)
+        assert e.value.status == native.MH_ERR_ARG
+        assert where in str(e.value) and what in str(e.value), str(e.value)
+    long_q = np.frombuffer(bytes(rng.choice(list(b"ACGT"), handle.max_read_len + 1)
+                                 .astype(np.uint8)), np.uint8)
+    with pytest.raises(native.MirageError) as e:
+        call([0], [[]], long_q, np.array([0, len(long_q)], np.uint64))
+    assert e.value.status == native.MH_ERR_RANGE and "pair 0" in str(e.value)
+    ix.free()
