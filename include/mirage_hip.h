@@ -239,6 +239,30 @@ mh_status mh_classify_host(mh_handle *h, const mh_index *idx, const uint8_t *h_r
                            mh_cand *h_cands, uint64_t cand_cap, uint64_t *top_needed,
                            uint64_t *cand_needed);
 
+/* mh_classify without allocation.  No counterpart in the reference: this
+ * entry point is for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Arguments, results and errors are mh_classify's; instead of the handle's
+ * growing scratch arena the call uses a fixed device array of
+ * MH_NOALLOC_SCRATCH_BYTES that the library holds on each device it runs on
+ * until the process ends.  At most MH_NOALLOC_MAX_READS reads per call;
+ * more, or a batch whose scratch does not fit (long maximum read lengths,
+ * MH_CLASSIFY_PREPROBE), returns MH_ERR_RANGE.  The HBM overflow tiers run
+ * on as many workspaces as fit (fewer than mh_classify uses: slower, same
+ * results), and tie sets larger than the fixed tie pool return
+ * MH_ERR_OVERFLOW rather than growing it.  MH_CLASSIFY_GRAPH is refused.
+ * Every call on a device shares that device's array: calls that overlap on
+ * one device (from different handles or threads) race, so serialise them.
+ * The array keeps the last call's contents; the call clears what must start
+ * at zero itself. */
+#define MH_NOALLOC_SCRATCH_BYTES 268435456ull /* 256 MiB */
+#define MH_NOALLOC_MAX_READS 1048576u
+mh_status mh_classify_noalloc(mh_handle *h, const mh_index *idx, const uint8_t *d_reads,
+                              const uint64_t *d_read_off, uint32_t n_reads, uint32_t flags,
+                              mh_read_result *d_out, uint32_t *d_top_pool, uint64_t top_cap,
+                              mh_cand *d_cands, uint64_t cand_cap, uint64_t *top_needed,
+                              uint64_t *cand_needed);
+
 /* Counters of the last mh_classify / mh_classify_host call (synchronises). */
 mh_status mh_get_counters(mh_handle *h, mh_counters *out);
 /* Kernel timings of the last call made with MH_CLASSIFY_TIMING (synchronises). */

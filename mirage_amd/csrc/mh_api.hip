@@ -126,6 +126,32 @@ bool valid_tiles(const mh_tile *t, uint32_t n, uint32_t max_len) {
   return t[n - 1].max_len >= max_len;
 }
 
+// Scratch of mh_classify_noalloc: a fixed device array instead of the
+// handle's arena, for callers that cannot allocate.  Each device the library
+// runs on holds its own copy until the process ends.  In host code the
+// array's name is not a device address: kernels and copies use the address
+// hipGetSymbolAddress gives for the current device (g_noalloc_scratch).
+static __device__ __align__(256) unsigned char scratch[MH_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+unsigned char *g_noalloc_scratch[kMaxDevices] = {};
+
+// Looks up the device address of `scratch` on h's device (the current one),
+// once per device.
+mh_status noalloc_init(mh_handle *h) {
+  if (h->device < 0 || h->device >= kMaxDevices)
+    return set_error(h, MH_ERR_ARG, "device number above the no-allocation table");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_scratch[h->device]) return MH_OK;
+  void *ptr = nullptr;
+  MH_HIP(h, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr || ((uintptr_t)ptr & 255))
+    return set_error(h, MH_ERR_HIP, "no-allocation scratch: bad symbol address");
+  g_noalloc_scratch[h->device] = (unsigned char *)ptr;
+  return MH_OK;
+}
+
 }  // namespace
 }  // namespace mh
 
@@ -158,6 +184,7 @@ mh_status mh_open(int32_t device, uint32_t max_read_len, mh_handle **out) {
   }
   if (!st && ensure_comp_table(device) != hipSuccess) st = MH_ERR_HIP;
   if (!st) st = build_luts(h);
+  if (!st) st = noalloc_init(h);
   if (!st) {
     std::vector<mh_tile> t = default_tiles(h->max_read_len);
     h->tiles = t;
@@ -256,7 +283,8 @@ static mh_status classify_on(mh_handle *h, Exec &x, const mh_index *idx, const u
                              const uint64_t *d_read_off, uint32_t n_reads, uint32_t flags,
                              mh_read_result *d_out, uint32_t *d_top_pool, uint64_t top_cap,
                              mh_cand *d_cands, uint64_t cand_cap, bool wait,
-                             uint64_t *top_needed, uint64_t *cand_needed) {
+                             uint64_t *top_needed, uint64_t *cand_needed,
+                             const FixedScratch *fixed = nullptr) {
   ClassifyArgs a;
   a.ix = idx->view();
   a.luts = h->luts();
@@ -269,7 +297,7 @@ static mh_status classify_on(mh_handle *h, Exec &x, const mh_index *idx, const u
   a.out.cand_cap = d_cands ? cand_cap : 0;
   a.final_out = d_out;
   mh_status st;
-  if (flags & MH_CLASSIFY_GRAPH) {
+  if ((flags & MH_CLASSIFY_GRAPH) && !fixed) {
     // Replay the captured batched call when every argument is the same;
     // otherwise capture it again.  The scratch arena is sized before the
     // capture (no allocation or synchronisation may happen inside it).
@@ -309,7 +337,7 @@ static mh_status classify_on(mh_handle *h, Exec &x, const mh_index *idx, const u
     MH_HIP(h, hipGraphLaunch(x.graph, x.stream));
     st = MH_OK;
   } else {
-    st = launch_classify(h, x, a, (flags & MH_CLASSIFY_BATCHED) != 0);
+    st = launch_classify(h, x, a, (flags & MH_CLASSIFY_BATCHED) != 0, fixed);
   }
   if (st) return st;
   if (!wait) return MH_OK;
@@ -318,7 +346,10 @@ static mh_status classify_on(mh_handle *h, Exec &x, const mh_index *idx, const u
   MH_HIP(h, hipStreamSynchronize(x.stream));
   if (top_needed) *top_needed = x.h_ctr->top_used;
   if (cand_needed) *cand_needed = x.h_ctr->cand_used;
-  if (x.h_ctr->tie_used > (unsigned long long)x.tie_factor * 2ull * n_reads + 256) {
+  if (fixed && x.h_ctr->tie_used > fixed->tie_cap)
+    return set_error(h, MH_ERR_OVERFLOW,
+                     "tie sets larger than the fixed scratch holds; classify fewer reads");
+  if (!fixed && x.h_ctr->tie_used > (unsigned long long)x.tie_factor * 2ull * n_reads + 256) {
     // Tie sets larger than the scratch pool: grow it for the next call.
     while ((unsigned long long)x.tie_factor * 2ull * n_reads + 256 < x.h_ctr->tie_used)
       x.tie_factor *= 2;
@@ -343,6 +374,40 @@ mh_status mh_classify(mh_handle *h, const mh_index *idx, const uint8_t *d_reads,
                              d_top_pool, top_cap, d_cands, cand_cap,
                              top_needed != nullptr || cand_needed != nullptr, top_needed,
                              cand_needed);
+  h->last.queries = 2ull * n_reads;
+  return st;
+}
+
+mh_status mh_classify_noalloc(mh_handle *h, const mh_index *idx, const uint8_t *d_reads,
+                              const uint64_t *d_read_off, uint32_t n_reads, uint32_t flags,
+                              mh_read_result *d_out, uint32_t *d_top_pool, uint64_t top_cap,
+                              mh_cand *d_cands, uint64_t cand_cap, uint64_t *top_needed,
+                              uint64_t *cand_needed) {
+  if (!h || !idx || (n_reads && (!d_reads || !d_read_off || !d_out))) return MH_ERR_ARG;
+  if (idx->owner != h && idx->device != h->device) return set_error(h, MH_ERR_ARG, "index device");
+  if (flags & MH_CLASSIFY_GRAPH)
+    return set_error(h, MH_ERR_ARG, "mh_classify_noalloc: MH_CLASSIFY_GRAPH is not supported");
+  if (n_reads > MH_NOALLOC_MAX_READS)
+    return set_error(h, MH_ERR_RANGE,
+                     "mh_classify_noalloc: " + std::to_string(n_reads) + " reads, at most " +
+                         std::to_string(MH_NOALLOC_MAX_READS) + " per call");
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->device);
+  h->last_valid = false;
+  unsigned char *base = g_noalloc_scratch[h->device];
+  if (!base) return set_error(h, MH_ERR_STATE, "mh_classify_noalloc: scratch not set up");
+  FixedScratch fs;
+  if (plan_fixed_scratch(h, n_reads, idx->threshold / 2, flags, MH_NOALLOC_SCRATCH_BYTES, fs))
+    return set_error(h, MH_ERR_RANGE,
+                     "mh_classify_noalloc: " + std::to_string(n_reads) +
+                         " reads do not fit the fixed scratch of " +
+                         std::to_string(MH_NOALLOC_SCRATCH_BYTES >> 20) +
+                         " MiB at this maximum read length");
+  fs.base = base;
+  mh_status st = classify_on(h, h->ex, idx, d_reads, d_read_off, n_reads, flags, d_out,
+                             d_top_pool, top_cap, d_cands, cand_cap,
+                             top_needed != nullptr || cand_needed != nullptr, top_needed,
+                             cand_needed, &fs);
   h->last.queries = 2ull * n_reads;
   return st;
 }

@@ -2567,7 +2567,8 @@ struct Layout {
 };
 
 Layout layout(uint32_t n_reads, uint32_t max_read_len, int s, uint32_t tie_factor,
-              uint32_t rec_stride) {
+              uint32_t rec_stride, uint32_t hbm_blocks = kHbmBlocks,
+              uint32_t big_blocks = kBigBlocks) {
   const size_t nq = 2ull * n_reads;
   Layout L;
   size_t o = 0;
@@ -2590,8 +2591,8 @@ Layout layout(uint32_t n_reads, uint32_t max_read_len, int s, uint32_t tie_facto
                                    (uint32_t *)nullptr, (int)(n_reads + 1));
   L.scan = take(L.scan_bytes);
   L.rec = take(nq * rec_stride * 8ull);
-  L.big = take(std::max(kHbmBlocks * ws_stride(hbm_caps(max_read_len, s, kHbmEcap)),
-                        kBigBlocks * ws_stride(hbm_caps(max_read_len, s, kBigEcap))));
+  L.big = take(std::max(hbm_blocks * ws_stride(hbm_caps(max_read_len, s, kHbmEcap)),
+                        big_blocks * ws_stride(hbm_caps(max_read_len, s, kBigEcap))));
   L.total = o;
   return L;
 }
@@ -2602,17 +2603,56 @@ size_t classify_scratch_bytes(const mh_handle *h, uint32_t n_reads, int s, uint3
   return layout(n_reads, h->max_read_len, s, tie_factor, stride).total;
 }
 
-mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool batched) {
+mh_status plan_fixed_scratch(const mh_handle *h, uint32_t n_reads, int s, uint32_t flags,
+                             size_t bytes, FixedScratch &fs) {
+  const uint32_t stride = probe_stride(active_tiles(h), s, h->lds_per_block, flags);
+  // Per-read parts with the smallest tie pool, and one workspace of each
+  // HBM tier: the least a call needs.
+  const Layout least = layout(n_reads, h->max_read_len, s, 4, stride, 1, 1);
+  if (least.total > bytes) return MH_ERR_RANGE;
+  const size_t hs = ws_stride(hbm_caps(h->max_read_len, s, kHbmEcap));
+  const size_t bs = ws_stride(hbm_caps(h->max_read_len, s, kBigEcap));
+  const size_t region = bytes - least.big;  // least.big: bytes before the HBM-tier region
+  // HBM-tier workspaces take what the per-read parts leave (they run one
+  // tier after the other in one region), up to the allocating path's counts.
+  fs.hbm_blocks = (uint32_t)std::min<size_t>(kHbmBlocks, region / hs);
+  fs.big_blocks = (uint32_t)std::min<size_t>(kBigBlocks, region / bs);
+  if (!fs.hbm_blocks || !fs.big_blocks) return MH_ERR_RANGE;
+  // Then the largest tie pool that fits, at most the allocating path's
+  // largest default.
+  fs.tie_factor = 4;
+  while (fs.tie_factor < 64 &&
+         layout(n_reads, h->max_read_len, s, fs.tie_factor * 2, stride, fs.hbm_blocks,
+                fs.big_blocks).total <= bytes)
+    fs.tie_factor *= 2;
+  fs.bytes = bytes;
+  fs.tie_cap = (size_t)fs.tie_factor * 2ull * n_reads + 256;
+  return MH_OK;
+}
+
+mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool batched,
+                          const FixedScratch *fixed) {
   hipError_t e = hipSuccess;
   const uint32_t n = a.rd.n;
   const uint64_t nq = 2ull * n;
   const std::vector<mh_tile> tiles = active_tiles(h);
   const uint32_t rec_stride =
       batched ? probe_stride(tiles, a.ix.s, h->lds_per_block, a.flags) : 0u;
-  const Layout L = layout(n, h->max_read_len, a.ix.s, x.tie_factor, rec_stride);
-  mh_status st = ensure_scratch(h, x, L.total);
-  if (st) return st;
-  unsigned char *base = (unsigned char *)x.d_scratch;
+  const uint32_t hbm_blocks = fixed ? fixed->hbm_blocks : kHbmBlocks;
+  const uint32_t big_blocks = fixed ? fixed->big_blocks : kBigBlocks;
+  const Layout L = layout(n, h->max_read_len, a.ix.s, fixed ? fixed->tie_factor : x.tie_factor,
+                          rec_stride, hbm_blocks, big_blocks);
+  unsigned char *base;
+  if (fixed) {
+    // A caller-provided region of fixed size: nothing is allocated.
+    if (L.total > fixed->bytes)
+      return set_error(h, MH_ERR_RANGE, "fixed scratch smaller than the call's layout");
+    base = fixed->base;
+  } else {
+    mh_status st = ensure_scratch(h, x, L.total);
+    if (st) return st;
+    base = (unsigned char *)x.d_scratch;
+  }
   QueryOut *qo = (QueryOut *)(base + L.qo);
   uint32_t *overflow = (uint32_t *)(base + L.overflow);
   uint32_t *overflow2 = (uint32_t *)(base + L.overflow2);
@@ -2763,11 +2803,11 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
       Ctx ch = c;
       ch.out.overflow = list[1];
       ch.out.overflow_cnt = &d->tier_n[3];
-      hipLaunchKernelGGL(k_classify_hbm<256>, dim3(kHbmBlocks), dim3(256), 0, s, ch, ca, big_ws,
+      hipLaunchKernelGGL(k_classify_hbm<256>, dim3(hbm_blocks), dim3(256), 0, s, ch, ca, big_ws,
                          ws_stride(ca), (const uint32_t *)list[0],
                          (const unsigned long long *)&d->tier_n[2], &d->tier_head[2], 0);
       const Caps cb = hbm_caps(h->max_read_len, a.ix.s, kBigEcap);
-      hipLaunchKernelGGL(k_classify_hbm<1024>, dim3(kBigBlocks), dim3(1024), 0, s, c, cb, big_ws,
+      hipLaunchKernelGGL(k_classify_hbm<1024>, dim3(big_blocks), dim3(1024), 0, s, c, cb, big_ws,
                          ws_stride(cb), (const uint32_t *)list[1],
                          (const unsigned long long *)&d->tier_n[3], &d->tier_head[3], 1);
     }

@@ -219,7 +219,23 @@ struct ClassifyArgs {
   Outputs out;
   mh_read_result *final_out;
 };
-mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool batched);
+// A scratch region of fixed size that a classify call uses instead of the
+// execution context's arena (nothing is allocated): the HBM tiers run on as
+// many workspaces as fit, and the tie pool is as large as fits.
+struct FixedScratch {
+  unsigned char *base = nullptr;
+  size_t bytes = 0;
+  uint32_t hbm_blocks = 0;  // workspaces of the first HBM tier
+  uint32_t big_blocks = 0;  // workspaces of the last-resort tier
+  uint32_t tie_factor = 4;  // tie pool entries per strand
+  size_t tie_cap = 0;       // tie pool entries
+};
+// Fills fs (all but base) for a call over n_reads reads within `bytes`;
+// MH_ERR_RANGE when the call cannot fit.
+mh_status plan_fixed_scratch(const mh_handle *h, uint32_t n_reads, int s, uint32_t flags,
+                             size_t bytes, FixedScratch &fs);
+mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool batched,
+                          const FixedScratch *fixed = nullptr);
 // Complement table in constant memory (once per device, before any capture).
 hipError_t ensure_comp_table(int device);
 // Scratch bytes one classify call needs.

@@ -37,6 +37,8 @@ MH_CLASSIFY_PREPROBE = 32
 MH_READ_BADBASE = 1
 MH_READ_TOO_LONG = 2
 MH_READ_CAPACITY = 4
+MH_NOALLOC_SCRATCH_BYTES = 256 << 20
+MH_NOALLOC_MAX_READS = 1 << 20
 
 
 class IndexInfo(ctypes.Structure):
@@ -93,8 +95,8 @@ class RequestInfo(ctypes.Structure):
 # Every symbol declared in include/mirage_hip.h.
 EXPORTS = ("mh_open", "mh_close", "mh_last_error", "mh_stream", "mh_max_read_len",
            "mh_max_batch_reads", "mh_index_build", "mh_index_free", "mh_index_get_info",
-           "mh_index_bucket", "mh_classify", "mh_classify_host", "mh_get_counters", "mh_get_timing",
-           "mh_score_seeds", "mh_set_tiles", "mh_get_tiles", "mh_autotune", "mh_server_start",
+           "mh_index_bucket", "mh_classify", "mh_classify_host", "mh_classify_noalloc",
+           "mh_get_counters", "mh_get_timing", "mh_score_seeds", "mh_set_tiles", "mh_get_tiles", "mh_autotune", "mh_server_start",
            "mh_server_submit", "mh_server_wait", "mh_server_stop", "mh_mhm_open", "mh_mhm_close",
            "mh_mhm_get_info", "mh_mhm_references", "mh_mhm_bucket", "mh_index_from_mhm",
            "mh_index_save_mhm", "mh_write_results", "mh_fasta_parse")
@@ -131,6 +133,7 @@ def load():
         "mh_index_bucket": (i32, [P, P, u32, P, P, u64, ctypes.POINTER(ctypes.c_int64)]),
         "mh_classify": (i32, [P, P, P, P, u32, u32, P, P, u64, P, u64, P, P]),
         "mh_classify_host": (i32, [P, P, P, P, u32, u32, P, P, u64, P, u64, P, P]),
+        "mh_classify_noalloc": (i32, [P, P, P, P, u32, u32, P, P, u64, P, u64, P, P]),
         "mh_get_counters": (i32, [P, ctypes.POINTER(Counters)]),
         "mh_get_timing": (i32, [P, ctypes.POINTER(Timing)]),
         "mh_score_seeds": (i32, [P, P, P, P, P, P, P, P, u32, u32, P, P, P, P, P]),
@@ -427,6 +430,21 @@ class DeviceIndex:
         torch.cuda.synchronize()
         types = (np.float32, np.float32, np.uint64, np.uint64, np.uint32)
         return tuple(o.cpu().numpy().view(t)[:n] for o, t in zip(outs, types))
+
+    def classify_noalloc(self, d_reads, d_read_off, n_reads, d_out, d_top, top_cap, flags=0,
+                         d_cands=None, cand_cap=0, wait=True):
+        """mh_classify_noalloc: the device path with the library's fixed
+        scratch array, no allocation.  Same arguments as classify_device."""
+        _torch_sync(d_reads)
+        lib, h = self.handle.lib, self.handle.h
+        tn, cn = ctypes.c_uint64(), ctypes.c_uint64()
+        st = lib.mh_classify_noalloc(h, self.ix, _dev_ptr(d_reads), _dev_ptr(d_read_off), n_reads,
+                                     flags, _dev_ptr(d_out), _dev_ptr(d_top), top_cap,
+                                     _dev_ptr(d_cands), cand_cap,
+                                     ctypes.byref(tn) if wait else None,
+                                     ctypes.byref(cn) if wait else None)
+        self.handle.check(st, "mh_classify_noalloc")
+        return (tn.value, cn.value) if wait else None
 
     def classify_device(self, d_reads, d_read_off, n_reads, d_out, d_top, top_cap, flags=0,
                         d_cands=None, cand_cap=0, wait=True, sync_inputs=True):
