@@ -45,6 +45,8 @@
 #include <unordered_map>
 #include <vector>
 
+#include <immintrin.h>
+
 namespace {
 
 constexpr uint32_t kHashBits = 28;                  // Hashmap.h:111
@@ -287,6 +289,192 @@ extern "C" mo_index *mo_index_build(const uint8_t *ref_bytes,
     ix->dir.assign((size_t)1 << kHashBits, 0);
     for (size_t b = 0; b < ix->keys.size(); b++) ix->dir[ix->keys[b]] = (uint32_t)(b + 1);
   }
+  return ix;
+}
+
+namespace {
+
+/* FNV-1 28-bit hashes of the 8 windows p[0..k), p[1..k+1), ..., p[7..k+7):
+ * the same arithmetic as mo_hash28, eight independent chains at once.  With
+ * AVX2 the chains are the lanes of one register (the 8 bytes p[j..j+7] are
+ * zero-extended into them); otherwise plain loops.  Reads p[0..k+7). */
+__attribute__((target("avx2"))) void hash28_x8_avx2(const uint8_t *p, int k, uint32_t *out) {
+  __m256i h = _mm256_set1_epi32((int)2166136261u);
+  const __m256i prime = _mm256_set1_epi32(16777619);
+  for (int j = 0; j < k; j++) {
+    uint64_t w;
+    std::memcpy(&w, p + j, 8);
+    const __m256i b = _mm256_cvtepu8_epi32(_mm_cvtsi64_si128((long long)w));
+    h = _mm256_xor_si256(_mm256_mullo_epi32(h, prime), b);
+  }
+  const __m256i f = _mm256_and_si256(_mm256_xor_si256(_mm256_srli_epi32(h, kHashBits), h),
+                                     _mm256_set1_epi32((int)kHashMask));
+  _mm256_storeu_si256((__m256i *)out, f);
+}
+
+void hash28_x8_plain(const uint8_t *p, int k, uint32_t *out) {
+  uint32_t h[8];
+  for (int t = 0; t < 8; t++) h[t] = 2166136261u;
+  for (int j = 0; j < k; j++)
+    for (int t = 0; t < 8; t++) h[t] = (h[t] * 16777619u) ^ p[j + t];
+  for (int t = 0; t < 8; t++) out[t] = ((h[t] >> kHashBits) ^ h[t]) & kHashMask;
+}
+
+}  // namespace
+
+/*
+ * The index restricted to the buckets a read set probes: for every hash that
+ * some probe window of the reads (both strands, windows j = 0, s, 2s, ... <
+ * m - T, Analyser.cpp:1459-1476) produces, the bucket holds exactly what
+ * mo_index_build gives it; every other hash is absent.  Classifying those
+ * reads (and only those) then gives the same results as the full index,
+ * without holding the entries of buckets no probe reads.  The reference
+ * windows are scanned as in mo_index_build, including the "N"-window cutoff,
+ * bucket 0's (ref 0, pos 0) entries for the windows dropped after a cutoff,
+ * and, with MO_FLAG_LAST_BUCKET_EMPTY, the emptied bucket of the highest hash
+ * of the whole index (kept only if a probe reads it).  The fragment rule
+ * needs every bucket's size, so T < 20 is refused (NULL).
+ */
+extern "C" mo_index *mo_index_build_for_reads(const uint8_t *ref_bytes, const uint64_t *ref_off,
+                                              uint32_t n_refs, int32_t threshold, uint32_t flags,
+                                              const uint8_t *reads, const uint64_t *read_off,
+                                              uint32_t n_reads) {
+  if (threshold < 20 || (!ref_bytes && n_refs) || !ref_off || (!reads && n_reads) ||
+      (!read_off && n_reads))
+    return nullptr;
+  mo_index *ix = new mo_index();
+  ix->threshold = threshold;
+  ix->stride = threshold / 2;                     // Hashmap.cpp:2789
+  ix->k = threshold - ix->stride + 1;             // Hashmap.cpp:2790
+  const int k = ix->k;
+  const uint32_t T = (uint32_t)threshold, s = (uint32_t)ix->stride;
+  const int threads = hw_threads(0);
+
+  // Hashes the reads probe, as a 2^28-bit set.
+  std::vector<std::atomic<uint64_t>> probed((size_t)1 << (kHashBits - 6));
+  for (auto &w : probed) w.store(0, std::memory_order_relaxed);
+  std::atomic<int> bad(0);
+  parallel_for(n_reads, threads, [&](uint64_t r, int) {
+    const uint64_t m = read_off[r + 1] - read_off[r];
+    if (m <= T) return;
+    std::vector<uint8_t> q(2 * m);
+    const uint8_t *rd = reads + read_off[r];
+    for (uint64_t i = 0; i < m; i++) q[i] = to_upper(rd[i]);
+    for (uint64_t i = 0; i < m; i++) {
+      const uint8_t c = complement(q[m - 1 - i]);
+      if (!c) {
+        bad.store(1);
+        return;
+      }
+      q[m + i] = c;
+    }
+    for (int strand = 0; strand < 2; strand++)
+      for (uint64_t j = 0; j < m - T; j += s) {
+        const uint32_t h = mo_hash28(q.data() + strand * m + j, k);
+        probed[h >> 6].fetch_or(1ull << (h & 63), std::memory_order_relaxed);
+      }
+  });
+  auto is_probed = [&](uint32_t h) {
+    return (probed[h >> 6].load(std::memory_order_relaxed) >> (h & 63)) & 1;
+  };
+
+  const uint64_t total = ref_off[n_refs] - ref_off[0];
+  ix->bytes.resize(total);
+  parallel_for((total + (1u << 24) - 1) >> 24, threads, [&](uint64_t c, int) {
+    const uint64_t a = c << 24, b = std::min<uint64_t>(total, a + (1u << 24));
+    for (uint64_t i = a; i < b; i++) ix->bytes[i] = to_upper(ref_bytes[ref_off[0] + i]);
+  });
+  ix->ref_off.resize(n_refs + 1);
+  for (uint32_t r = 0; r <= n_refs; r++) ix->ref_off[r] = ref_off[r] - ref_off[0];
+
+  std::vector<uint8_t> nnn(k, 'N');
+  const uint32_t h_nnn = mo_hash28(nnn.data(), k);  // Hashmap.cpp:2834
+  ix->cutoff.assign(n_refs, 0);
+  const bool avx2 = __builtin_cpu_supports("avx2");
+  // Per thread: kept entries and the largest hash of any indexed window.
+  std::vector<std::vector<Entry>> kept(threads);
+  std::vector<uint32_t> top(threads, 0);
+  std::vector<char> any(threads, 0);
+  parallel_for(n_refs, threads, [&](uint64_t r, int t) {
+    const uint64_t n = ix->ref_off[r + 1] - ix->ref_off[r];
+    if (n <= (uint64_t)k) return;                 // undefined in the reference
+    const uint64_t stop = n - k;                  // Hashmap.cpp:2839-2841
+    const uint8_t *p = ix->bytes.data() + ix->ref_off[r];
+    uint32_t hv[8];
+    uint64_t i = 0;
+    bool cut = false;
+    uint32_t top_r = 0;  // kept in registers: the per-thread slots share cache lines
+    std::vector<Entry> &out = kept[t];
+    while (i < stop && !cut) {
+      const uint64_t nb = std::min<uint64_t>(8, stop - i);
+      if (nb == 8) {  // windows i..i+7 end at i + 7 + k <= stop - 1 + k < n
+        if (avx2) hash28_x8_avx2(p + i, k, hv);
+        else hash28_x8_plain(p + i, k, hv);
+      } else {
+        for (uint64_t t2 = 0; t2 < nb; t2++) hv[t2] = mo_hash28(p + i + t2, k);
+      }
+      for (uint64_t t2 = 0; t2 < nb; t2++) {
+        const uint32_t h = hv[t2];
+        if (h == h_nnn) {  // the reference stops indexing this one here
+          cut = true;
+          break;
+        }
+        top_r = std::max(top_r, h);
+        if (is_probed(h)) out.push_back(Entry{h, (uint32_t)r, (uint32_t)i});
+        i++;  // window i + 1 is hv[t2 + 1]
+      }
+    }
+    ix->cutoff[r] = i;
+    if (i > 0) {
+      any[t] = 1;
+      top[t] = std::max(top[t], top_r);
+    }
+  });
+  uint64_t phantom = 0;
+  for (uint32_t r = 0; r < n_refs; r++) {
+    const uint64_t n = ix->ref_off[r + 1] - ix->ref_off[r];
+    if (n > (uint64_t)k) phantom += (n - k) - ix->cutoff[r];
+  }
+  bool have = phantom > 0;
+  uint32_t highest = 0;  // the highest hash of the full index
+  for (int t = 0; t < threads; t++)
+    if (any[t]) {
+      highest = have ? std::max(highest, top[t]) : top[t];
+      have = true;
+    }
+  size_t n_kept = 0;
+  for (auto &v : kept) n_kept += v.size();
+  const bool zero = is_probed(0) && phantom > 0;
+  ix->entries.reserve(n_kept + (zero ? phantom : 0));
+  for (auto &v : kept) {
+    ix->entries.insert(ix->entries.end(), v.begin(), v.end());
+    std::vector<Entry>().swap(v);
+  }
+  if (zero)
+    for (uint64_t i = 0; i < phantom; i++) ix->entries.push_back(Entry{0u, 0u, 0u});
+  std::sort(ix->entries.begin(), ix->entries.end(), [](const Entry &a, const Entry &c) {
+    if (a.hash != c.hash) return a.hash < c.hash;
+    if (a.ref != c.ref) return a.ref < c.ref;
+    return a.pos < c.pos;
+  });
+  for (uint64_t i = 0; i < ix->entries.size(); i++)
+    if (i == 0 || ix->entries[i].hash != ix->entries[i - 1].hash) {
+      ix->keys.push_back(ix->entries[i].hash);
+      ix->key_begin.push_back(i);
+    }
+  ix->key_begin.push_back(ix->entries.size());
+  if ((flags & MO_FLAG_LAST_BUCKET_EMPTY) && have && !ix->keys.empty() &&
+      ix->keys.back() == highest) {
+    // The highest hash of the whole index keeps its key, not its entries.
+    const size_t b = ix->keys.size() - 1;
+    ix->entries.resize(ix->key_begin[b]);
+    ix->key_begin[b + 1] = ix->key_begin[b];
+  }
+  if (ix->entries.size() > (1u << 22)) {
+    ix->dir.assign((size_t)1 << kHashBits, 0);
+    for (size_t b = 0; b < ix->keys.size(); b++) ix->dir[ix->keys[b]] = (uint32_t)(b + 1);
+  }
+  (void)bad;
   return ix;
 }
 

@@ -57,6 +57,15 @@ uint32_t mo_hash28(const uint8_t *p, int32_t len);
  * offsets.  Returns NULL on bad arguments. */
 mo_index *mo_index_build(const uint8_t *ref_bytes, const uint64_t *ref_off,
                          uint32_t n_refs, int32_t threshold, uint32_t flags);
+/* The index restricted to the buckets that the reads' probe windows (both
+ * strands) hash to: those buckets are exactly mo_index_build's, every other
+ * hash is absent, so classifying these reads gives the full index's results.
+ * For large references, where the full index does not fit host memory.
+ * T < 20 (the fragment rule needs every bucket's size) returns NULL. */
+mo_index *mo_index_build_for_reads(const uint8_t *ref_bytes, const uint64_t *ref_off,
+                                   uint32_t n_refs, int32_t threshold, uint32_t flags,
+                                   const uint8_t *reads, const uint64_t *read_off,
+                                   uint32_t n_reads);
 void mo_index_free(mo_index *idx);
 uint64_t mo_index_num_entries(const mo_index *idx);
 uint64_t mo_index_num_buckets(const mo_index *idx);

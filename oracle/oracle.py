@@ -55,6 +55,9 @@ def lib():
         L.mo_hash28.argtypes = [P, ctypes.c_int32]
         L.mo_index_build.restype = P
         L.mo_index_build.argtypes = [P, P, ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint32]
+        L.mo_index_build_for_reads.restype = P
+        L.mo_index_build_for_reads.argtypes = [P, P, ctypes.c_uint32, ctypes.c_int32,
+                                               ctypes.c_uint32, P, P, ctypes.c_uint32]
         L.mo_index_free.argtypes = [P]
         L.mo_index_num_entries.restype = ctypes.c_uint64
         L.mo_index_num_entries.argtypes = [P]
@@ -129,6 +132,28 @@ class Index:
                                        threshold, flags)
         if not self._h:
             raise ValueError("oracle index build failed")
+
+    @classmethod
+    def for_reads(cls, ref_bytes, ref_off, threshold, reads, read_off,
+                  last_bucket_empty: bool = True):
+        """The index restricted to the buckets these reads probe
+        (mo_index_build_for_reads): the same results for these reads as the
+        full index, for references too large to index whole on the host.
+        T >= 20 only."""
+        self = cls.__new__(cls)
+        self.ref_bytes = np.ascontiguousarray(ref_bytes, dtype=np.uint8)
+        self.ref_off = np.ascontiguousarray(ref_off, dtype=np.uint64)
+        self.threshold = threshold
+        self.n_refs = len(self.ref_off) - 1
+        r = np.ascontiguousarray(reads, dtype=np.uint8)
+        ro = np.ascontiguousarray(read_off, dtype=np.uint64)
+        flags = MO_FLAG_LAST_BUCKET_EMPTY if last_bucket_empty else 0
+        self._h = lib().mo_index_build_for_reads(_ptr(self.ref_bytes), _ptr(self.ref_off),
+                                                 self.n_refs, threshold, flags, _ptr(r), _ptr(ro),
+                                                 len(ro) - 1)
+        if not self._h:
+            raise ValueError("oracle index build for reads failed (T < 20?)")
+        return self
 
     def __del__(self):
         if getattr(self, "_h", None):

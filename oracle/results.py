@@ -20,7 +20,11 @@ The references of a group come out in the order two library sorts leave
 them: std::sort of the read's scored candidates by score, descending
 (:404), then std::stable_sort by taxonomy occurrence with a comparator that
 puts taxonomy id -1 first whatever the other element is (:266-293).
-Both are restated in oracle/stdsort.py.  A read's scored candidates are
+Neither is a plain ordering (std::sort is not stable; the comparator is not
+a strict weak ordering), so both are called in the library itself, built
+into the C++ oracle with the reference's comparators (mo_std_sort_desc,
+mo_std_stable_sort_tax in oracle/mirage_oracle.cpp; GNU libstdc++ of the
+image's GCC 11, the reference's toolchain).  A read's scored candidates are
 those with Score > -1 on its surviving strand, in candidate order, i.e.
 ascending reference id (MatrixCPU.cpp:323-370).
 """
@@ -28,7 +32,7 @@ from collections import Counter
 
 import numpy as np
 
-from . import stdsort
+from . import oracle as O
 
 KINDS = ("Raw", "HighSensitivity", "HighPrecision", "Balanced")
 HEADER = "InputID;InputSequence;Confident;TaxIDs;Occurence_Per_TaxID;Headers\n"
@@ -41,11 +45,21 @@ def to_string(x):
     return "%f" % float(np.float32(x))
 
 
+def sort_by_score_desc(scores):
+    """std::sort of [(rel, ref)] by rel, descending (Analyser.cpp:404)."""
+    return O.std_sort_desc(scores)
+
+
+def stable_sort_by_taxonomy(refs, taxids):
+    """std::stable_sort of reference ids with the comparator of
+    Analyser.cpp:268-293 (taxonomy id -1 first; then higher occurrence)."""
+    return O.std_stable_sort_tax(refs, taxids)
+
+
 def score_groups(scores, first_only=True):
     """scores: [(rel as float32, ref)] in candidate order.  Returns
     [(rel, [refs])] as LogMatrixResult groups them (Analyser.cpp:401-424)."""
-    a = list(scores)
-    stdsort.sort(a, lambda l, r: l[0] > r[0])
+    a = [(np.float32(r), int(i)) for r, i in sort_by_score_desc(list(scores))]
     out = [(a[0][0], [])]
     for rel, ref in a:
         if rel != out[-1][0]:
@@ -63,25 +77,7 @@ def line(input_id, read_header, rel, refs, ref_headers, taxids):
         return s + "?;"
     tax = [taxids[r] for r in refs]
     occ = Counter(tax)
-
-    def comp(a, b):  # :268-293, as written
-        ta, tb = taxids[a], taxids[b]
-        if ta == -1:
-            return True
-        if tb == -1:
-            return False
-        if ta == tb:
-            return False
-        if occ[ta] == occ[tb]:
-            return False
-        if occ[ta] > occ[tb]:
-            return True
-        if occ[ta] < occ[tb]:
-            return False
-        return ta < tb
-
-    res = list(refs)
-    stdsort.stable_sort(res, comp)
+    res = [int(r) for r in stable_sort_by_taxonomy(list(refs), taxids)]
     cur = taxids[res[0]]
     max_occ = occ[cur]
     for column in ("tax", "occ"):  # :299-351; cur carries over between them

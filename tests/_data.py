@@ -23,6 +23,12 @@ def golden_names():
 def load_golden(name):
     with gzip.open(os.path.join(GOLDEN, f"ref_{name}.json.gz"), "rt") as f:
         fx = json.load(f)
+    if "refs" not in fx:  # only the FASTA texts are stored: parse them
+        from oracle import fasta as F
+        fx["refs"] = [x.decode("latin-1") for x in
+                      F.parse(fx["reference_fasta"].encode("latin-1"))[0]]
+        fx["reads"] = [x.decode("latin-1") for x in
+                       F.parse(fx["query_fasta"].encode("latin-1"))[0]]
     rb, ro = pack(fx["refs"])
     qb, qo = pack(fx["reads"])
     return fx, rb, ro, qb, qo

@@ -97,6 +97,23 @@ def datasets():
            D.ReadSpec(n=200, lengths=L(mode=2, a=40, b=300), p_exact=0.2, p_random=0.1,
                       sub_rate=0.01, rc_frac=0.5),
            112, None)
+    # Odd T (k = s + 2): the reference's shipped default is T = 31
+    # (MirAge/data/mirage.identifier.cfg:22).
+    yield ("t31_uniform", 31,
+           D.RefSpec(n=24, lengths=L(mode=0, a=6000)),
+           D.ReadSpec(n=400, lengths=L(mode=0, a=150), p_exact=0.05, p_random=0.1,
+                      sub_rate=0.01, rc_frac=0.5),
+           117, None)
+    yield ("t31_mixed_indels", 31,
+           D.RefSpec(n=30, lengths=L(mode=1, a=8000, b=0.7, lo=2000, cap=50000), gc_lo=0.4,
+                     gc_hi=0.6, seg_pool=4, seg_lo=300, seg_hi=1200, seg_frac=0.4),
+           D.ReadSpec(n=300, lengths=L(mode=3, a=60, b=300, c=900, d=0.6, mix=0.7, lo=60,
+                                       cap=2400),
+                      sub_rate=0.02, indel_rate=0.005, rc_frac=0.5),
+           118, None)
+    # The reference's own sample data (MirAge/data/reference.fasta and
+    # query.fasta) at its shipped T = 31.
+    yield ("t31_shipped_data", 31, None, None, 0, "shipped")
     yield ("t30_tiny_nruns", 30,
            D.RefSpec(n=6, lengths=L(mode=0, a=2000)),
            D.ReadSpec(n=200, lengths=L(mode=0, a=150), sub_rate=0.01, rc_frac=0.5),
@@ -282,7 +299,17 @@ def write_fasta(path, prefix, by, off):
                 f.write(s[j:j + 80] + b"\n")
 
 
-def run_reference(workdir, threshold):
+SHIPPED_DIR = "/root/reference/MirAge/data"
+
+
+def ref_id_of(header, names):
+    """Reference index of a header printed in results_Raw.log: the generated
+    FASTA names references r<i>; `names` maps other names to indices."""
+    name = header.lstrip(">").split()[0]
+    return names[name] if names is not None else int(name[1:])
+
+
+def run_reference(workdir, threshold, names=None):
     if os.path.exists(workdir):
         shutil.rmtree(workdir)
     os.makedirs(os.path.join(workdir, "mirage", "data"))
@@ -329,7 +356,7 @@ def run_reference(workdir, threshold):
                 continue
             cols = line.split(";")
             rid = int(cols[0])
-            refs = sorted(int(h.lstrip(">").split()[0][1:]) for h in cols[5].split(","))
+            refs = sorted(ref_id_of(h, names) for h in cols[5].split(","))
             res[rid] = [cols[2], refs]
     return res
 
@@ -339,7 +366,20 @@ def main():
     for name, T, rspec, qspec, seed, edits in datasets():
         if only and name not in only:
             continue
-        if edits == "engineered":
+        names = None
+        if edits == "shipped":
+            from oracle import fasta as F
+            texts = tuple(open(os.path.join(SHIPPED_DIR, fn), "rb").read()
+                          for fn in ("reference.fasta", "query.fasta"))
+            (rseq, rhead), (qseq, _) = F.parse(texts[0]), F.parse(texts[1])
+            names = {h[1:-1].decode().split()[0]: i for i, h in enumerate(rhead)}
+            rb = np.frombuffer(b"".join(rseq), np.uint8).copy()
+            ro = np.zeros(len(rseq) + 1, np.uint64)
+            ro[1:] = np.cumsum([len(x) for x in rseq])
+            qb = np.frombuffer(b"".join(qseq), np.uint8).copy()
+            qo = np.zeros(len(qseq) + 1, np.uint64)
+            qo[1:] = np.cumsum([len(x) for x in qseq])
+        elif edits == "engineered":
             rb, ro, qb, qo = engineered(seed, T)
         elif edits == "ties20":
             rb, ro, qb, qo = ties20(seed)
@@ -353,8 +393,13 @@ def main():
                 apply_edits(edits, rb, ro, qb, qo, rng)
         base = os.path.join(RUNS, name)
         os.makedirs(base, exist_ok=True)
-        texts = None
-        if edits == "fasta_forms":
+        if edits != "shipped":
+            texts = None
+        if edits == "shipped":
+            for fn, t in zip(("reference.fasta", "query.fasta"), texts):
+                with open(os.path.join(base, fn), "wb") as f:
+                    f.write(t)
+        elif edits == "fasta_forms":
             frng = np.random.default_rng(seed + 1)
             texts = (fasta_forms_text("r", rb, ro, frng, True),
                      fasta_forms_text("q", qb, qo, frng, False))
@@ -364,8 +409,8 @@ def main():
         else:
             write_fasta(os.path.join(base, "reference.fasta"), "r", rb, ro)
             write_fasta(os.path.join(base, "query.fasta"), "q", qb, qo)
-        r1 = run_reference(os.path.join(base, "a"), T)
-        r2 = run_reference(os.path.join(base, "b"), T)
+        r1 = run_reference(os.path.join(base, "a"), T, names)
+        r2 = run_reference(os.path.join(base, "b"), T, names)
         mhm = [open(r.pop("_mhm"), "rb").read() for r in (r1, r2)]
         logs = [r.pop("_logs") for r in (r1, r2)]
         if r1 != r2 or mhm[0] != mhm[1] or logs[0] != logs[1]:
@@ -388,6 +433,9 @@ def main():
         if texts is not None:  # the FASTA files as the reference read them
             fx["reference_fasta"] = texts[0].decode("latin-1")
             fx["query_fasta"] = texts[1].decode("latin-1")
+        if edits == "shipped":
+            # the sequences are those of the texts (tests/_data.py parses them)
+            del fx["refs"], fx["reads"]
         with gzip.GzipFile(os.path.join(OUT_DIR, f"ref_{name}.json.gz"), "wb", mtime=0) as f:
             f.write(json.dumps(fx).encode())
         print(f"{name}: {len(r1)} classified of {len(qo) - 1}")

@@ -123,12 +123,14 @@ def test_rejected_texts(handle, text, what):
     assert e.value.status == -1 and what in str(e.value)
 
 
-def test_fasta_to_logs_on_device(handle, tmp_path):
-    """The reference and query FASTA of t22_fasta_forms parsed on the device;
+@pytest.mark.parametrize("name", ["t22_fasta_forms", "t31_shipped_data"])
+def test_fasta_to_logs_on_device(handle, tmp_path, name):
+    """The reference and query FASTA of a dataset parsed on the device (the
+    forms test, and the reference's own sample data at its shipped T = 31);
     the parsed reads classified where they lie; the logs written from the
     results and the parsed headers are the reference's."""
     import torch
-    fx = _data.load_golden("t22_fasta_forms")[0]
+    fx = _data.load_golden(name)[0]
     rt = to_device(fx["reference_fasta"].encode("latin-1"))
     qt = to_device(fx["query_fasta"].encode("latin-1"))
     rseq, rso, rhdr, rho = handle.parse_fasta(rt)
@@ -152,4 +154,7 @@ def test_fasta_to_logs_on_device(handle, tmp_path):
                          [rb_[rho_[i]:rho_[i + 1]] for i in range(len(rho_) - 1)], cands=cands)
     for k in ("Raw", "HighSensitivity", "HighPrecision", "Balanced"):
         assert (tmp_path / f"results_{k}.log").read_text(encoding="latin-1") == fx["logs"][k], k
+    if name == "t31_shipped_data":
+        assert (tmp_path / "results_Raw.log").read_text().split("\n")[1] == \
+            "0;>RM2|S2|R14/1;0.627418;-1;2;>2,>1"
     ix.free()

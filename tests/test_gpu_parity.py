@@ -135,3 +135,24 @@ def test_index_buckets_match_oracle(handle):
         if h == 0xFFFFFFFF:
             continue
         assert ix.bucket(int(h)) == oix.bucket(int(h)), f"bucket {h:#x}"
+
+
+@pytest.mark.parametrize("name", ["t30_gc_lognormal", "t31_mixed_indels"])
+@pytest.mark.parametrize("flags", [0, native.MH_CLASSIFY_BATCHED])
+def test_ultrafast_matches_oracle(handle, name, flags):
+    """UltraFast mode (MatrixCPU.cpp:69: no head/tail extension, a seed
+    counts its own length only) against the oracle's, per read and per
+    candidate."""
+    fx, rb, ro, qb, qo = _data.load_golden(name)
+    ix = handle.build_index(rb, ro, fx["threshold"])
+    out, top, cands = ix.classify_host(qb, qo, flags=flags | native.MH_CLASSIFY_ULTRAFAST,
+                                       want_cands=True)
+    oout, otop, ocands = O.Index(rb, ro, fx["threshold"]).classify(qb, qo, want_cands=True,
+                                                                   ultrafast=True)
+    compare_reads(out, top, oout, otop, f"{name} ultrafast")
+    compare_cands(cands, ocands, f"{name} ultrafast")
+    # the mode matters on these reads: extension changes some scores
+    eout, _, _ = O.Index(rb, ro, fx["threshold"]).classify(qb, qo)
+    assert not np.array_equal(bits(eout["highest"]), bits(oout["highest"]))
+    assert handle.counters()["ext_compares"] == 0
+    ix.free()

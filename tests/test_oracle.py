@@ -200,3 +200,61 @@ def test_relative_score_and_hash_match_reference():
         assert f"{fbits(O.relative_score(qs, rs, sc)):08x}" == line.strip()
     for s, line in zip(seqs, out[500:]):
         assert O.hash28(s) == int(line)
+
+
+def _same_results(a, b):
+    return (np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32))
+            and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]))
+
+
+@pytest.mark.parametrize("name", [n for n in _data.golden_names()
+                                  if int(n.split("_")[0][1:]) >= 20])
+def test_index_for_reads_gives_full_index_results(name):
+    """The index restricted to the probed buckets (used where the full
+    index does not fit host memory) classifies the reads exactly as the
+    full index does, per read and per candidate, on every golden with
+    T >= 20 (N runs, bucket 0's copies, shared segments, odd T)."""
+    fx, rb, ro, qb, qo = _data.load_golden(name)
+    full = O.Index(rb, ro, fx["threshold"])
+    part = O.Index.for_reads(rb, ro, fx["threshold"], qb, qo)
+    assert part.num_entries < full.num_entries
+    a = full.classify(qb, qo, want_cands=True)
+    b = part.classify(qb, qo, want_cands=True)
+    assert _same_results(a, b)
+
+
+def test_index_for_reads_buckets_and_last_bucket():
+    """Bucket for bucket equal to the full index on every probed hash (and
+    absent elsewhere); the highest hash of the whole index is present but
+    empty when a read probes it (MO_FLAG_LAST_BUCKET_EMPTY); bucket 0's
+    dropped-window copies survive; T < 20 is refused."""
+    rng = np.random.default_rng(21)
+    refs = [bytes(rng.choice(list(b"ACGT"), n).tolist()) for n in (3000, 2500, 4000)]
+    refs[1] = refs[1][:900] + b"N" * 30 + refs[1][930:]  # a cutoff: bucket 0 copies
+    T = 30
+    s, k = T // 2, T - T // 2 + 1
+    rb, ro = _data.pack([r.decode() for r in refs])
+    full = O.Index(rb, ro, T)
+    # the window with the highest hash of the whole index, found by brute force
+    best = max(((O.hash28(seq[i:i + k]), r, i) for r, seq in enumerate(refs)
+                for i in range(full.cutoff(r))), key=lambda t: t[0])
+    top_read = refs[best[1]][best[2]:best[2] + k] + b"ACGT" * 10
+    reads = [top_read] + [refs[r][a:a + 150] for r, a in ((0, 100), (1, 700), (2, 2000))]
+    reads.append(b"A" * k + b"C" * 40)  # probes hashes the references lack
+    qb, qo = _data.pack([x.decode() for x in reads])
+    part = O.Index.for_reads(rb, ro, T, qb, qo)
+    assert full.bucket(best[0]) == [] and part.bucket(best[0]) == []
+    comp = bytes.maketrans(b"ACGT", b"TGCA")
+    probed = set()
+    for x in reads:
+        for strand in (x, x.translate(comp)[::-1]):
+            for j in range(0, len(strand) - T, s):
+                probed.add(O.hash28(strand[j:j + k]))
+    for h in probed:
+        assert part.bucket(h) == full.bucket(h), h
+    unprobed = [O.hash28(refs[2][i:i + k]) for i in range(3000, 3050)]
+    assert all(part.bucket(h) is None for h in unprobed if h not in probed)
+    assert _same_results(full.classify(qb, qo, want_cands=True),
+                         part.classify(qb, qo, want_cands=True))
+    with pytest.raises(ValueError):
+        O.Index.for_reads(rb, ro, 18, qb, qo)

@@ -11,7 +11,6 @@ import pytest
 from mirage_amd import native
 from oracle import oracle as O
 from oracle import results as R
-from oracle import stdsort
 from tests import _data
 
 NAMES = _data.golden_names()
@@ -54,33 +53,6 @@ def written(tmp_path):
     return {k: (tmp_path / f"results_{k}.log").read_text() for k in R.KINDS}
 
 
-def test_stdsort_matches_libstdcxx():
-    rng = np.random.default_rng(1)
-    for _ in range(1500):
-        n, k = int(rng.integers(0, 120)), int(rng.integers(1, 6))
-        sc = [(np.float32(rng.integers(0, k)) / 4, i) for i in range(n)]
-        a = list(sc)
-        stdsort.sort(a, lambda l, r: l[0] > r[0])
-        assert [(float(x), y) for x, y in a] == [(float(x), y) for x, y in O.std_sort_desc(sc)]
-        nt = int(rng.integers(1, 40))
-        tax = rng.integers(-1, 4, nt).tolist()
-        ids = rng.permutation(nt)[:int(rng.integers(0, nt + 1))].tolist()
-        occ = {t: sum(tax[r] == t for r in ids) for t in set(tax)}
-
-        def comp(a_, b_):
-            ta, tb = tax[a_], tax[b_]
-            if ta == -1:
-                return True
-            if tb == -1:
-                return False
-            if ta == tb or occ[ta] == occ[tb]:
-                return False
-            return occ[ta] > occ[tb]
-        c = list(ids)
-        stdsort.stable_sort(c, comp)
-        assert c == O.std_stable_sort_tax(ids, tax)
-
-
 @pytest.mark.parametrize("name", NAMES)
 def test_oracle_logs_match_reference(name):
     fx, n, n_refs, out, top, cands = classified(name)
@@ -93,15 +65,26 @@ def test_planted_sort_mistakes_are_caught():
     """A stable score sort, or a stable taxonomy sort with a valid
     comparator, changes the reference order of tied references."""
     fx, n, n_refs, out, top, cands = classified("t30_ties20")
-    orig_sort, orig_stable = stdsort.sort, stdsort.stable_sort
+    orig_sort, orig_stable = R.sort_by_score_desc, R.stable_sort_by_taxonomy
     try:
-        stdsort.sort = lambda a, comp: a.sort(key=lambda t: -t[0])
+        R.sort_by_score_desc = lambda a: sorted(a, key=lambda t: -t[0])
         assert oracle_logs(fx, out, cands)["Raw"] != fx["logs"]["Raw"]
-        stdsort.sort = orig_sort
-        stdsort.stable_sort = lambda a, comp: None
+        R.sort_by_score_desc = orig_sort
+        R.stable_sort_by_taxonomy = lambda refs, taxids: list(refs)
         assert oracle_logs(fx, out, cands)["Raw"] != fx["logs"]["Raw"]
     finally:
-        stdsort.sort, stdsort.stable_sort = orig_sort, orig_stable
+        R.sort_by_score_desc, R.stable_sort_by_taxonomy = orig_sort, orig_stable
+
+
+def test_shipped_sample_line():
+    """The reference's own sample data (MirAge/data/reference.fasta and
+    query.fasta) at its shipped T = 31 (mirage.identifier.cfg:22): the oracle
+    prints the line the reference binary printed, read 1 unclassified."""
+    fx, n, n_refs, out, top, cands = classified("t31_shipped_data")
+    assert fx["threshold"] == 31 and n == 2 and n_refs == 2
+    logs = oracle_logs(fx, out, cands)
+    assert logs["Raw"] == R.HEADER + "0;>RM2|S2|R14/1;0.627418;-1;2;>2,>1\n"
+    assert out["n_top"][1] == 0
 
 
 @pytest.mark.parametrize("name", NAMES)
