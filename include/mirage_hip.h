@@ -349,9 +349,22 @@ typedef struct {
 } mh_tile;
 
 /* Installs the bucket table used by MH_CLASSIFY_BATCHED (at most 8 rows,
- * ascending max_len; the last row must cover mh_max_read_len). */
+ * ascending max_len; the last row must cover mh_max_read_len).  Without one
+ * (or after mh_set_tiles(h, NULL, 0)) each call uses defaults sized for its
+ * index: the entry capacities follow the index's entries per bucket. */
 mh_status mh_set_tiles(mh_handle *h, const mh_tile *tiles, uint32_t n_tiles);
-mh_status mh_get_tiles(const mh_handle *h, mh_tile *tiles, uint32_t cap, uint32_t *n_tiles);
+/* The length tiles (and, when tiers is not NULL, the two LDS overflow
+ * tiers) that a batched call on idx uses. */
+mh_status mh_get_tiles(const mh_handle *h, const mh_index *idx, mh_tile *tiles, uint32_t cap,
+                       uint32_t *n_tiles, mh_tile *tiers);
+/* The two LDS overflow tiers that strands too large for their length tile
+ * pass through before the HBM-workspace tiers: tier 0 (second chance) holds
+ * reads up to tiers[0].max_len with tiers[0].entries_cap entries (a strand
+ * with more entries skips both LDS tiers), tier 1 the longer reads.  Block
+ * 64..1024, entries_cap a power of two >= 32, max_len 64..max_read_len.
+ * NULL restores the defaults (sized from the first length tile).  Results
+ * do not depend on them. */
+mh_status mh_set_overflow_tiles(mh_handle *h, const mh_tile *tiers);
 /* Times candidate tiles per length bucket on the given device-resident reads
  * and installs the fastest (the project's autotuner). */
 mh_status mh_autotune(mh_handle *h, const mh_index *idx, const uint8_t *d_reads,

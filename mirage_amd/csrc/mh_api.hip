@@ -185,10 +185,6 @@ mh_status mh_open(int32_t device, uint32_t max_read_len, mh_handle **out) {
   if (!st && ensure_comp_table(device) != hipSuccess) st = MH_ERR_HIP;
   if (!st) st = build_luts(h);
   if (!st) st = noalloc_init(h);
-  if (!st) {
-    std::vector<mh_tile> t = default_tiles(h->max_read_len);
-    h->tiles = t;
-  }
   if (st) {
     mh_close(h);
     return st;
@@ -312,8 +308,7 @@ static mh_status classify_on(mh_handle *h, Exec &x, const mh_index *idx, const u
         hipGraphExecDestroy(x.graph);
         x.graph = nullptr;
       }
-      st = ensure_scratch(h, x, classify_scratch_bytes(h, n_reads, idx->threshold / 2,
-                                                       x.tie_factor, a.flags));
+      st = ensure_scratch(h, x, classify_scratch_bytes(h, a.ix, n_reads, x.tie_factor, a.flags));
       if (st) return st;
       hipGraph_t g = nullptr;
       MH_HIP(h, hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));
@@ -397,7 +392,7 @@ mh_status mh_classify_noalloc(mh_handle *h, const mh_index *idx, const uint8_t *
   unsigned char *base = g_noalloc_scratch[h->device];
   if (!base) return set_error(h, MH_ERR_STATE, "mh_classify_noalloc: scratch not set up");
   FixedScratch fs;
-  if (plan_fixed_scratch(h, n_reads, idx->threshold / 2, flags, MH_NOALLOC_SCRATCH_BYTES, fs))
+  if (plan_fixed_scratch(h, idx->view(), n_reads, flags, MH_NOALLOC_SCRATCH_BYTES, fs))
     return set_error(h, MH_ERR_RANGE,
                      "mh_classify_noalloc: " + std::to_string(n_reads) +
                          " reads do not fit the fixed scratch of " +
@@ -531,6 +526,12 @@ mh_status mh_score_seeds(mh_handle *h, const mh_index *idx, const uint8_t *d_que
 
 mh_status mh_set_tiles(mh_handle *h, const mh_tile *tiles, uint32_t n_tiles) {
   if (!h) return MH_ERR_ARG;
+  if (!tiles && !n_tiles) {
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->tiles.clear();
+    h->tiles_version++;
+    return MH_OK;
+  }
   if (!tiles || !valid_tiles(tiles, n_tiles, h->max_read_len))
     return set_error(h, MH_ERR_ARG,
                      "mh_set_tiles: 1-8 tiles of ascending max_len, the last at least "
@@ -542,10 +543,41 @@ mh_status mh_set_tiles(mh_handle *h, const mh_tile *tiles, uint32_t n_tiles) {
   return MH_OK;
 }
 
-mh_status mh_get_tiles(const mh_handle *h, mh_tile *tiles, uint32_t cap, uint32_t *n_tiles) {
-  if (!h || !n_tiles) return MH_ERR_ARG;
-  *n_tiles = (uint32_t)h->tiles.size();
-  for (uint32_t i = 0; i < cap && i < h->tiles.size(); i++) tiles[i] = h->tiles[i];
+mh_status mh_set_overflow_tiles(mh_handle *h, const mh_tile *tiers) {
+  if (!h) return MH_ERR_ARG;
+  if (!tiers) {
+    std::lock_guard<std::mutex> lk(h->mu);
+    h->tier_tiles.clear();
+    h->tiles_version++;
+    return MH_OK;
+  }
+  for (int i = 0; i < 2; i++) {
+    const mh_tile &t = tiers[i];
+    const uint32_t b = t.block;
+    if ((b != 64 && b != 128 && b != 256 && b != 512 && b != 1024) || t.entries_cap < 32 ||
+        (t.entries_cap & (t.entries_cap - 1)) || t.max_len < 64 || t.max_len > h->max_read_len)
+      return set_error(h, MH_ERR_ARG,
+                       "mh_set_overflow_tiles: two tiles; block 64..1024 (a power of two), "
+                       "entries_cap a power of two >= 32, max_len 64..max_read_len");
+  }
+  std::lock_guard<std::mutex> lk(h->mu);
+  h->tier_tiles.assign(tiers, tiers + 2);
+  h->tiles_version++;
+  return MH_OK;
+}
+
+mh_status mh_get_tiles(const mh_handle *h, const mh_index *idx, mh_tile *tiles, uint32_t cap,
+                       uint32_t *n_tiles, mh_tile *tiers) {
+  if (!h || !idx || !n_tiles) return MH_ERR_ARG;
+  const IndexView v = idx->view();
+  const std::vector<mh_tile> t = active_tiles(h, v);
+  *n_tiles = (uint32_t)t.size();
+  for (uint32_t i = 0; i < cap && i < t.size(); i++) tiles[i] = t[i];
+  if (tiers) {
+    const std::vector<mh_tile> tt = tier_tiles(h, v);
+    tiers[0] = tt[0];
+    tiers[1] = tt[1];
+  }
   return MH_OK;
 }
 

@@ -33,7 +33,7 @@ extern "C" mh_status mh_autotune(mh_handle *h, const mh_index *idx, const uint8_
   std::vector<mh_tile> tiles;
   {
     std::lock_guard<std::mutex> lk(h->mu);
-    tiles = h->tiles.empty() ? default_tiles(h->max_read_len) : h->tiles;
+    tiles = active_tiles(h, idx->view());
   }
   const uint32_t blocks[] = {16, 32, 64, 128, 256, 512};
   const uint32_t ecaps[] = {32, 64, 128, 256, 512, 1024, 2048, 4096};

@@ -2523,27 +2523,62 @@ size_t ws_stride(const Caps &c) { return (ws_bytes(c) + 255) & ~size_t(255); }
 
 }  // namespace
 
-std::vector<mh_tile> default_tiles(uint32_t max_read_len) {
-  // Entry capacities follow the expected bucket entries of each length
-  // (about two per probe on a 5000-reference index at k = 16); short reads
-  // run 2 or 4 to a wavefront.
+std::vector<mh_tile> default_tiles(uint32_t max_read_len, int T, uint32_t epp_x16) {
+  // Entry capacities follow the expected bucket entries of each length: the
+  // base table fits about two entries per probe (a 5000-reference index at
+  // k = 16), and short reads run 2 or 4 to a wavefront.  An index with more
+  // entries per bucket widens a tile whose expected entries (the probes of
+  // a read of 3/4 its longest length times the index's entries per bucket,
+  // plus 8) exceed the base capacity: capacity the next power of two, at
+  // least 64 threads per 256 entries.  (On 8e9 entries, 29.8 per bucket, the
+  // 150 bp reads at T = 38 take 64-thread teams of 256 entries, the fastest
+  // measured: 542 ms for 10M reads against 1,012 with 16-lane teams of 32.)
   const mh_tile base[7] = {{200, 16, 32, 0},    {400, 32, 64, 0},    {1024, 128, 512, 0},
                            {2048, 128, 512, 0},  {4096, 256, 1024, 0}, {8192, 512, 1024, 0},
                            {0xFFFFFFFFu, 1024, 2048, 0}};
+  const uint32_t s = (uint32_t)std::max(1, T / 2);
   std::vector<mh_tile> t;
   for (mh_tile b : base) {
-    if (b.max_len >= max_read_len) {
-      b.max_len = max_read_len;
-      t.push_back(b);
-      break;
+    const bool last = b.max_len >= max_read_len;
+    if (last) b.max_len = max_read_len;
+    if (T > 0 && epp_x16) {
+      const uint32_t len = b.max_len / 4 * 3;
+      const uint64_t probes = len > (uint32_t)T ? (len - (uint32_t)T) / s + 1 : 1;
+      const uint64_t want = (probes * epp_x16 + 15) / 16 + 8;
+      if (want > b.entries_cap) {
+        uint32_t cap = b.entries_cap;
+        while (cap < want && cap < 4096) cap *= 2;
+        b.entries_cap = cap;
+        b.block = std::max<uint32_t>(b.block, std::min<uint32_t>(1024, cap / 4));
+      }
     }
     t.push_back(b);
+    if (last) break;
   }
   return t;
 }
 
-std::vector<mh_tile> active_tiles(const mh_handle *h) {
-  return h->tiles.empty() ? default_tiles(h->max_read_len) : h->tiles;
+std::vector<mh_tile> default_tier_tiles(uint32_t max_read_len, uint32_t first_cap) {
+  // The second-chance tier holds four times the first tile's entries (at
+  // least 512), a 64-thread wavefront per 256 entries: on 8e9 entries the
+  // reads of shared segments (about 900 entries per strand at 150 bp) then
+  // stay in LDS (1,024 entries, 256 threads: 385 against 543 ms for 10M
+  // reads with the 512-entry tier, whose overflow went to HBM workspaces).
+  const uint32_t cap0 = std::max<uint32_t>(512, std::min<uint32_t>(4096, 4 * first_cap));
+  return {mh_tile{std::min<uint32_t>(1024u, max_read_len), std::max<uint32_t>(128, cap0 / 4),
+                  cap0, 0},
+          mh_tile{max_read_len, 256, 4096, 0}};
+}
+
+uint32_t entries_per_bucket_x16(const IndexView &ix) { return ix.epp_x16; }
+
+std::vector<mh_tile> active_tiles(const mh_handle *h, const IndexView &ix) {
+  return h->tiles.empty() ? default_tiles(h->max_read_len, ix.T, ix.epp_x16) : h->tiles;
+}
+
+std::vector<mh_tile> tier_tiles(const mh_handle *h, const IndexView &ix) {
+  if (h->tier_tiles.size() == 2) return h->tier_tiles;
+  return default_tier_tiles(h->max_read_len, active_tiles(h, ix)[0].entries_cap);
 }
 
 // Probe records per strand of the first length tile (0: that tile stages,
@@ -2597,15 +2632,16 @@ Layout layout(uint32_t n_reads, uint32_t max_read_len, int s, uint32_t tie_facto
   return L;
 }
 
-size_t classify_scratch_bytes(const mh_handle *h, uint32_t n_reads, int s, uint32_t tie_factor,
-                              uint32_t flags) {
-  const uint32_t stride = probe_stride(active_tiles(h), s, h->lds_per_block, flags);
-  return layout(n_reads, h->max_read_len, s, tie_factor, stride).total;
+size_t classify_scratch_bytes(const mh_handle *h, const IndexView &ix, uint32_t n_reads,
+                              uint32_t tie_factor, uint32_t flags) {
+  const uint32_t stride = probe_stride(active_tiles(h, ix), ix.s, h->lds_per_block, flags);
+  return layout(n_reads, h->max_read_len, ix.s, tie_factor, stride).total;
 }
 
-mh_status plan_fixed_scratch(const mh_handle *h, uint32_t n_reads, int s, uint32_t flags,
-                             size_t bytes, FixedScratch &fs) {
-  const uint32_t stride = probe_stride(active_tiles(h), s, h->lds_per_block, flags);
+mh_status plan_fixed_scratch(const mh_handle *h, const IndexView &ix, uint32_t n_reads,
+                             uint32_t flags, size_t bytes, FixedScratch &fs) {
+  const int s = ix.s;
+  const uint32_t stride = probe_stride(active_tiles(h, ix), s, h->lds_per_block, flags);
   // Per-read parts with the smallest tie pool, and one workspace of each
   // HBM tier: the least a call needs.
   const Layout least = layout(n_reads, h->max_read_len, s, 4, stride, 1, 1);
@@ -2635,7 +2671,7 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
   hipError_t e = hipSuccess;
   const uint32_t n = a.rd.n;
   const uint64_t nq = 2ull * n;
-  const std::vector<mh_tile> tiles = active_tiles(h);
+  const std::vector<mh_tile> tiles = active_tiles(h, a.ix);
   const uint32_t rec_stride =
       batched ? probe_stride(tiles, a.ix.s, h->lds_per_block, a.flags) : 0u;
   const uint32_t hbm_blocks = fixed ? fixed->hbm_blocks : kHbmBlocks;
@@ -2724,8 +2760,7 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
       // On 2M mixed-length reads this takes the overflow tiers from 10.5 to
       // 9.2 ms.  The middle tier keeps strands whose entries fit the
       // second-chance tier but whose read is longer than its 1,024 bases.
-      c.lds_ecap_max = tile_caps(mh_tile{std::min<uint32_t>(1024u, h->max_read_len), 128, 512, 0},
-                                 a.ix.s, h->lds_per_block).ecap;
+      c.lds_ecap_max = tile_caps(tier_tiles(h, a.ix)[0], a.ix.s, h->lds_per_block).ecap;
       const uint32_t nb = (uint32_t)std::min<size_t>(tiles.size(), 8);
       Lens d_maxlen{};
       for (uint32_t i = 0; i < nb; i++) d_maxlen.v[i] = tiles[i].max_len;
@@ -2778,15 +2813,13 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
     }
     // Overflow tiers, each a persistent kernel over the list the one before
     // it filled (two lists in turn):
-    //   0  second chance: 128-thread teams, 1024 bases, 512 entries
-    //   1  middle: 256-thread teams with the largest LDS workspace
+    //   0  second chance (LDS, default 128-thread teams, 1024 bases, 512 entries)
+    //   1  middle (LDS, default 256-thread teams with the largest workspace)
     //   2  HBM workspaces of 2^15 entries over kHbmBlocks workgroups
     //   3  last resort: HBM workspaces of 2^19 entries, flags what is left
     uint32_t *list[2] = {overflow, overflow2};
     DevCounters *d = x.d_ctr;
-    const mh_tile lds_tiers[2] = {
-        {std::min<uint32_t>(1024u, h->max_read_len), 128, 512, 0},
-        {h->max_read_len, 256, 4096, 0}};  // lds_ecap_max is tier 0's capacity
+    const std::vector<mh_tile> lds_tiers = tier_tiles(h, a.ix);  // lds_ecap_max: tier 0's cap
     for (int t = 0; t < 2; t++) {
       const Caps tc = tile_caps(lds_tiers[t], a.ix.s, h->lds_per_block);
       Ctx ct = c;

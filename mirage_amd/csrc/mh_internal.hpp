@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <atomic>
 #include <mutex>
 #include <string>
@@ -35,6 +36,9 @@ struct IndexView {
   // for every probe j that hit bucket 0; the kernels weight that seed instead
   // of storing the copies.
   uint32_t phantom;
+  // Stored entries per non-empty bucket, times 16 (what a probe that finds
+  // its bucket gathers on average); sizes the default tiles.
+  uint32_t epp_x16;
 };
 
 // Tables of glibc logf for the relative score (MatrixResult.cpp:21-29):
@@ -140,6 +144,9 @@ struct mh_index {
     v.s = threshold / 2;
     v.k = threshold - threshold / 2 + 1;
     v.phantom = (uint32_t)phantom;
+    v.epp_x16 = n_buckets ? (uint32_t)std::min<uint64_t>(
+                                0xFFFFFFFFull, (16 * (n_entries - phantom) + n_buckets / 2) / n_buckets)
+                          : 0;
     return v;
   }
 };
@@ -185,6 +192,7 @@ struct mh_handle {
   mh_counters last{};
   bool last_valid = false;
   std::vector<mh_tile> tiles;
+  std::vector<mh_tile> tier_tiles;  // the two LDS overflow tiers (empty: defaults)
   uint64_t tiles_version = 0;
   mh::Luts luts() const { return mh::Luts{d_x_log, d_smax_log, n_x, n_smax}; }
 };
@@ -232,15 +240,15 @@ struct FixedScratch {
 };
 // Fills fs (all but base) for a call over n_reads reads within `bytes`;
 // MH_ERR_RANGE when the call cannot fit.
-mh_status plan_fixed_scratch(const mh_handle *h, uint32_t n_reads, int s, uint32_t flags,
-                             size_t bytes, FixedScratch &fs);
+mh_status plan_fixed_scratch(const mh_handle *h, const IndexView &ix, uint32_t n_reads,
+                             uint32_t flags, size_t bytes, FixedScratch &fs);
 mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool batched,
                           const FixedScratch *fixed = nullptr);
 // Complement table in constant memory (once per device, before any capture).
 hipError_t ensure_comp_table(int device);
 // Scratch bytes one classify call needs.
-size_t classify_scratch_bytes(const mh_handle *h, uint32_t n_reads, int s, uint32_t tie_factor,
-                              uint32_t flags);
+size_t classify_scratch_bytes(const mh_handle *h, const IndexView &ix, uint32_t n_reads,
+                              uint32_t tie_factor, uint32_t flags);
 // Reads the counters of the last call on x (x's stream must be idle).
 void counters_from(const DevCounters &d, uint64_t n_reads, mh_counters &out);
 mh_status launch_score_seeds(mh_handle *h, const IndexView &ix, const Luts &luts,
@@ -250,6 +258,12 @@ mh_status launch_score_seeds(mh_handle *h, const IndexView &ix, const Luts &luts
                              uint64_t *d_sum_r, uint64_t *d_sum_c, uint32_t *d_hits);
 
 // Default tiles (mh_classify.hip).
-std::vector<mh_tile> default_tiles(uint32_t max_read_len);
+// Default length tiles for an index at threshold T with epp_x16 (IndexView)
+// entries per bucket; the LDS overflow tiers follow the first tile.  A table
+// installed with mh_set_tiles / mh_set_overflow_tiles replaces them.
+std::vector<mh_tile> default_tiles(uint32_t max_read_len, int T, uint32_t epp_x16);
+std::vector<mh_tile> default_tier_tiles(uint32_t max_read_len, uint32_t first_cap);
+std::vector<mh_tile> active_tiles(const mh_handle *h, const IndexView &ix);
+std::vector<mh_tile> tier_tiles(const mh_handle *h, const IndexView &ix);
 
 }  // namespace mh

@@ -100,8 +100,8 @@ void free_slot(Slot &s) {
   if (s.h_top) hipHostFree(s.h_top);
 }
 
-mh_status init_slot(mh_handle *h, Slot &s, uint64_t cap_bytes, uint64_t cap_reads, int stride,
-                    uint32_t flags) {
+mh_status init_slot(mh_handle *h, Slot &s, uint64_t cap_bytes, uint64_t cap_reads,
+                    const IndexView &ix, uint32_t flags) {
   mh_status st = exec_init(h, s.ex);
   if (st) return st;
   s.cap_bytes = cap_bytes;
@@ -115,7 +115,7 @@ mh_status init_slot(mh_handle *h, Slot &s, uint64_t cap_bytes, uint64_t cap_read
   MH_HIP(h, hipHostMalloc(&s.h_off, (cap_reads + 1) * 8, hipHostMallocDefault));
   MH_HIP(h, hipHostMalloc(&s.h_out, cap_reads * sizeof(mh_read_result), hipHostMallocDefault));
   MH_HIP(h, hipHostMalloc(&s.h_top, s.top_cap * 4, hipHostMallocDefault));
-  return ensure_scratch(h, s.ex, classify_scratch_bytes(h, (uint32_t)cap_reads, stride, 4,
+  return ensure_scratch(h, s.ex, classify_scratch_bytes(h, ix, (uint32_t)cap_reads, 4,
                                                         flags));
 }
 
@@ -298,7 +298,7 @@ mh_status mh_server_start(mh_handle *h, const mh_index *idx, const mh_server_con
       std::min<uint64_t>(sv->cfg.hbm_budget / sv->cfg.max_inflight, 512ull << 20);
   for (uint32_t i = 0; i < sv->cfg.max_inflight; i++) {
     sv->slots.emplace_back(new Slot());
-    mh_status st = init_slot(h, *sv->slots.back(), cap_bytes, cap_reads, idx->threshold / 2,
+    mh_status st = init_slot(h, *sv->slots.back(), cap_bytes, cap_reads, idx->view(),
                              sv->cfg.flags | MH_CLASSIFY_BATCHED);
     if (st) {
       for (auto &s : sv->slots) free_slot(*s);

@@ -96,7 +96,8 @@ class RequestInfo(ctypes.Structure):
 EXPORTS = ("mh_open", "mh_close", "mh_last_error", "mh_stream", "mh_max_read_len",
            "mh_max_batch_reads", "mh_index_build", "mh_index_free", "mh_index_get_info",
            "mh_index_bucket", "mh_classify", "mh_classify_host", "mh_classify_noalloc",
-           "mh_get_counters", "mh_get_timing", "mh_score_seeds", "mh_set_tiles", "mh_get_tiles", "mh_autotune", "mh_server_start",
+           "mh_get_counters", "mh_get_timing", "mh_score_seeds", "mh_set_tiles", "mh_get_tiles",
+           "mh_set_overflow_tiles", "mh_autotune", "mh_server_start",
            "mh_server_submit", "mh_server_wait", "mh_server_stop", "mh_mhm_open", "mh_mhm_close",
            "mh_mhm_get_info", "mh_mhm_references", "mh_mhm_bucket", "mh_index_from_mhm",
            "mh_index_save_mhm", "mh_write_results", "mh_fasta_parse")
@@ -138,7 +139,8 @@ def load():
         "mh_get_timing": (i32, [P, ctypes.POINTER(Timing)]),
         "mh_score_seeds": (i32, [P, P, P, P, P, P, P, P, u32, u32, P, P, P, P, P]),
         "mh_set_tiles": (i32, [P, P, u32]),
-        "mh_get_tiles": (i32, [P, P, u32, ctypes.POINTER(u32)]),
+        "mh_get_tiles": (i32, [P, P, P, u32, ctypes.POINTER(u32), P]),
+        "mh_set_overflow_tiles": (i32, [P, P]),
         "mh_autotune": (i32, [P, P, P, P, u32, u32]),
         "mh_server_start": (i32, [P, P, ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]),
         "mh_server_submit": (i32, [P, P, P, u32, ctypes.POINTER(u64)]),
@@ -277,17 +279,35 @@ class Handle:
         return seq[:ns.value], so[:r + 1], hdr[:nh.value], ho[:r + 1]
 
     def set_tiles(self, tiles):
+        """Installs a length-tile table (mh_set_tiles); None restores the
+        index-sized defaults."""
+        if tiles is None:
+            self.check(self.lib.mh_set_tiles(self.h, None, 0), "mh_set_tiles")
+            return
         arr = (Tile * len(tiles))(*[Tile(*t) for t in tiles])
         self.check(self.lib.mh_set_tiles(self.h, ctypes.cast(arr, ctypes.c_void_p), len(tiles)),
                    "mh_set_tiles")
 
-    def get_tiles(self):
+    def get_tiles(self, index):
+        """(length tiles, overflow tiers) a batched call on `index` uses."""
         arr = (Tile * 8)()
+        tiers = (Tile * 2)()
         n = ctypes.c_uint32()
-        self.check(self.lib.mh_get_tiles(self.h, ctypes.cast(arr, ctypes.c_void_p), 8,
-                                         ctypes.byref(n)), "mh_get_tiles")
-        return [(arr[i].max_len, arr[i].block, arr[i].entries_cap, arr[i].grid)
-                for i in range(n.value)]
+        self.check(self.lib.mh_get_tiles(self.h, index.ix, ctypes.cast(arr, ctypes.c_void_p), 8,
+                                         ctypes.byref(n), ctypes.cast(tiers, ctypes.c_void_p)),
+                   "mh_get_tiles")
+        as_tuple = lambda t: (t.max_len, t.block, t.entries_cap, t.grid)  # noqa: E731
+        return [as_tuple(arr[i]) for i in range(n.value)], [as_tuple(t) for t in tiers]
+
+    def set_overflow_tiles(self, tiers):
+        """The two LDS overflow tiers (mh_set_overflow_tiles); None restores
+        the defaults."""
+        if tiers is None:
+            self.check(self.lib.mh_set_overflow_tiles(self.h, None), "mh_set_overflow_tiles")
+            return
+        arr = (Tile * 2)(*[Tile(*t) for t in tiers])
+        self.check(self.lib.mh_set_overflow_tiles(self.h, ctypes.cast(arr, ctypes.c_void_p)),
+                   "mh_set_overflow_tiles")
 
     def timing(self):
         t = Timing()
@@ -300,7 +320,7 @@ class Handle:
         installs the fastest (mh_autotune).  Returns the tile table."""
         self.check(self.lib.mh_autotune(self.h, index.ix, _dev_ptr(d_reads), _dev_ptr(d_read_off),
                                         n_reads, iters), "mh_autotune")
-        return self.get_tiles()
+        return self.get_tiles(index)[0]
 
     def counters(self):
         c = Counters()

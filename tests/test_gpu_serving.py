@@ -68,13 +68,13 @@ def test_autotune_keeps_results(handle):
     rb, ro, qb, qo, _ = D.generate(w)
     ix = handle.build_index(rb, ro, w.threshold)
     before, btop, _ = ix.classify_host(qb, qo, flags=native.MH_CLASSIFY_BATCHED)
-    saved = handle.get_tiles()
+    defaults = handle.get_tiles(ix)[0]
     d_reads = torch.from_numpy(qb).cuda()
     d_off = torch.from_numpy(qo.view(np.int64)).cuda()
     torch.cuda.synchronize()
     tiles = handle.autotune(ix, d_reads, d_off, len(qo) - 1, iters=1)
-    assert len(tiles) == len(saved)
+    assert len(tiles) == len(defaults)
     after, atop, _ = ix.classify_host(qb, qo, flags=native.MH_CLASSIFY_BATCHED)
-    handle.set_tiles(saved)
+    handle.set_tiles(None)
     assert np.array_equal(before["highest"].view(np.uint32), after["highest"].view(np.uint32))
     assert native.tie_sets(before, btop) == native.tie_sets(after, atop)

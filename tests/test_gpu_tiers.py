@@ -84,7 +84,6 @@ TILE_TABLES = {
 def test_tiny_tiles_send_work_to_the_tiers(handle, flags, table):
     """With small tiles (whole-workgroup teams, or lane teams of 8, 16 and
     32 lanes) many strands overflow; results are unchanged."""
-    saved = handle.get_tiles()
     try:
         handle.set_tiles([(ml or handle.max_read_len, b, e, g)
                           for ml, b, e, g in TILE_TABLES[table]])
@@ -95,7 +94,7 @@ def test_tiny_tiles_send_work_to_the_tiers(handle, flags, table):
         out, top, cands = ix.classify_host(qb, qo, flags=flags, want_cands=True)
         assert handle.counters()["overflow"] > 0
     finally:
-        handle.set_tiles(saved)
+        handle.set_tiles(None)
     oix = O.Index(rb, ro, w.threshold)
     oout, otop, ocands = oix.classify(qb, qo, want_cands=True)
     compare_reads(out, top, oout, otop, "tiny tiles")
@@ -170,7 +169,10 @@ def test_preprobe_reads_at_unaligned_addresses(handle, shift):
 def test_invalid_tile_tables_are_refused(handle):
     """A tile table the kernels cannot run is refused with its reason, and
     the handle keeps its tiles."""
-    saved = handle.get_tiles()
+    w = D.workload(1, 0.01)
+    rb, ro, _, _, _ = D.generate(w)
+    ix = handle.build_index(rb, ro, w.threshold)
+    saved = handle.get_tiles(ix)
     bad = [[(handle.max_read_len, 16, 16, 0)],                      # entries_cap below 32
            [(handle.max_read_len, 24, 32, 0)],                      # block not a power of two
            [(400, 16, 32, 0), (200, 32, 64, 0)],                    # max_len not ascending
@@ -179,7 +181,8 @@ def test_invalid_tile_tables_are_refused(handle):
         with pytest.raises(native.MirageError) as e:
             handle.set_tiles(t)
         assert e.value.status == native.MH_ERR_ARG and "mh_set_tiles" in str(e.value), t
-    assert handle.get_tiles() == saved
+    assert handle.get_tiles(ix) == saved
+    ix.free()
 
 
 WORKGROUP_TABLES = {
@@ -217,7 +220,6 @@ def test_workgroup_teams_score_many_references(handle, flags, table):
             a = int(ro[r]) + int(rng.integers(0, 20000 - 150))
             reads.append(mutate(rng, rb[a:a + 150].copy(), 2))
     qb, qo = pack(reads)
-    saved = handle.get_tiles()
     try:
         handle.set_tiles([(ml or handle.max_read_len, b, e, g)
                           for ml, b, e, g in WORKGROUP_TABLES[table]])
@@ -225,7 +227,7 @@ def test_workgroup_teams_score_many_references(handle, flags, table):
         out, top, cands = ix.classify_host(qb, qo, flags=flags, want_cands=True, top_cap=64 * 240)
         ix.free()
     finally:
-        handle.set_tiles(saved)
+        handle.set_tiles(None)
     oix = O.Index(rb, ro, 30)
     oout, otop, ocands = oix.classify(qb, qo, want_cands=True, top_cap=64)
     compare_reads(out, top, oout, otop, f"workgroup teams {table}")
