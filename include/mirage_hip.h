@@ -124,6 +124,8 @@ typedef struct {
   float merge_ms;          /* strand merge */
   uint32_t n_buckets;
   float probe_ms;          /* probe pre-pass (MH_CLASSIFY_PREPROBE), else 0 */
+  float tier_ms[4];        /* big_ms per overflow tier: second chance, middle, HBM
+                              workspaces, last resort */
 } mh_timing;
 
 /* ---- handle ------------------------------------------------------------ */
@@ -265,6 +267,14 @@ mh_status mh_classify_noalloc(mh_handle *h, const mh_index *idx, const uint8_t *
 
 /* Counters of the last mh_classify / mh_classify_host call (synchronises). */
 mh_status mh_get_counters(mh_handle *h, mh_counters *out);
+/* The same counters per kernel of the last call (synchronises): rows 0-7
+ * the length tiles in table order (the plain form: row 0), rows 8-11 the
+ * overflow tiers (second chance, middle, HBM workspaces, last resort).  In
+ * each row `queries` counts the strands that kernel finished and the work
+ * fields their work; `overflow` counts, for a tier row, the strands that
+ * entered the tier.  *n receives the number of rows (12); up to cap are
+ * written. */
+mh_status mh_get_kernel_counters(mh_handle *h, mh_counters *out, uint32_t cap, uint32_t *n);
 /* Kernel timings of the last call made with MH_CLASSIFY_TIMING (synchronises). */
 mh_status mh_get_timing(mh_handle *h, mh_timing *out);
 

@@ -482,6 +482,18 @@ mh_status mh_get_counters(mh_handle *h, mh_counters *out) {
   return MH_OK;
 }
 
+mh_status mh_get_kernel_counters(mh_handle *h, mh_counters *out, uint32_t cap, uint32_t *n) {
+  if (!h || !n || (cap && !out)) return MH_ERR_ARG;
+  std::lock_guard<std::mutex> lk(h->mu);
+  hipSetDevice(h->device);
+  MH_HIP(h, hipMemcpyAsync(h->ex.h_ctr, h->ex.d_ctr, sizeof(DevCounters), hipMemcpyDeviceToHost,
+                           h->ex.stream));
+  MH_HIP(h, hipStreamSynchronize(h->ex.stream));
+  *n = (uint32_t)kKernelRows;
+  kernel_counters_from(*h->ex.h_ctr, out, std::min<uint32_t>(cap, (uint32_t)kKernelRows));
+  return MH_OK;
+}
+
 mh_status mh_get_timing(mh_handle *h, mh_timing *out) {
   if (!h || !out) return MH_ERR_ARG;
   std::lock_guard<std::mutex> lk(h->mu);
@@ -503,6 +515,13 @@ mh_status mh_get_timing(mh_handle *h, mh_timing *out) {
     prev = 1 + (int)i;
   }
   out->big_ms = ms(prev, 10);
+  // each tier from the end of the one before (the last length tile first)
+  int at = prev;
+  for (int t = 0; t < 4; t++) {
+    const int ev = t < 3 ? 13 + t : 10;
+    out->tier_ms[t] = ms(at, ev);
+    at = ev;
+  }
   if (x.timed_probe) out->probe_ms = ms(12, 9);
   out->merge_ms = ms(10, 11);
   return MH_OK;

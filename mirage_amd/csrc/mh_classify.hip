@@ -520,6 +520,7 @@ struct Ctx {
   // A strand with more is passed on flagged (kBigFlag), and the LDS tiers
   // pass it on unread to the HBM-workspace tier.
   uint32_t lds_ecap_max;
+  uint32_t kid;  // statistics row of this launch (kKernelRows)
   Outputs out;
 };
 
@@ -1902,12 +1903,21 @@ __device__ __forceinline__ void stats_begin(const Ws &ws) {
   TM::sync();
 }
 
-// Adds the team's statistics to slot `slot` of the device counters.
+// Adds the team's statistics to slot `slot` of the device counters and to
+// the launch's row.
 template <class TM>
-__device__ __forceinline__ void stats_flush(const Ws &ws, DevCounters *ctr, uint32_t slot) {
+__device__ __forceinline__ void stats_flush(const Ws &ws, DevCounters *ctr, uint32_t slot,
+                                            uint32_t kid) {
   TM::sync();
-  if (TM::rank() < kNumStats && ws.stats[TM::rank()])
+  if (TM::rank() < kNumStats && ws.stats[TM::rank()]) {
     atomicAdd(&ctr->stats[slot & (kStatSlots - 1)][TM::rank()], ws.stats[TM::rank()]);
+    atomicAdd(&ctr->kstats[kid < kKernelRows ? kid : 0][TM::rank()], ws.stats[TM::rank()]);
+  }
+}
+
+template <class TM>
+__device__ __forceinline__ void count_done(const Ws &ws, uint32_t r) {
+  if (r == kDone && TM::rank() == 0) ws.stats[kStrands] += 1;
 }
 
 template <class TM>
@@ -1956,9 +1966,11 @@ __attribute__((amdgpu_waves_per_eu(TM::kPerWave > 1 ? 4 : 1, 8))) void k_classif
   if (head == nullptr) {
     if (slot < nq) {
       const uint32_t q = qlist ? qlist[slot] : slot;
-      after_query<TM>(c, q, process_query<TM>(c, ws, cap, q, false, s_comp));
+      const uint32_t r = process_query<TM>(c, ws, cap, q, false, s_comp);
+      count_done<TM>(ws, r);
+      after_query<TM>(c, q, r);
     }
-    stats_flush<TM>(ws, c.out.ctr, slot);
+    stats_flush<TM>(ws, c.out.ctr, slot, c.kid);
     return;
   }
   const unsigned long long n = nq_dev ? *nq_dev : nq;
@@ -1983,10 +1995,12 @@ __attribute__((amdgpu_waves_per_eu(TM::kPerWave > 1 ? 4 : 1, 8))) void k_classif
         pass_on<TM>(c, q);
         continue;
       }
-      after_query<TM>(c, q, process_query<TM>(c, ws, cap, q, false, s_comp));
+      const uint32_t r = process_query<TM>(c, ws, cap, q, false, s_comp);
+      count_done<TM>(ws, r);
+      after_query<TM>(c, q, r);
     }
   }
-  stats_flush<TM>(ws, c.out.ctr, slot);
+  stats_flush<TM>(ws, c.out.ctr, slot, c.kid);
 }
 
 // Queries that overflowed the LDS tiers, with the workspace in HBM: one
@@ -2022,10 +2036,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __syncthreads();
     if (slot == 0xFFFFFFFFu) break;
     const uint32_t q = list[slot] & ~kBigFlag;
-    if (process_query<TM>(c, ws, cap, q, last_resort != 0, s_comp) != kDone)
-      pass_on<TM>(c, q);
+    const uint32_t r = process_query<TM>(c, ws, cap, q, last_resort != 0, s_comp);
+    count_done<TM>(ws, r);
+    if (r != kDone) pass_on<TM>(c, q);
   }
-  stats_flush<TM>(ws, c.out.ctr, blockIdx.x);
+  stats_flush<TM>(ws, c.out.ctr, blockIdx.x, c.kid);
 }
 
 // Probe records for the strands of the shortest length bucket.  An 8-lane
@@ -2710,6 +2725,7 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
   c.rec_stride = 0;
   c.rec_maxlen = 0;
   c.lds_ecap_max = 0;
+  c.kid = 0;
   c.out = a.out;
   c.out.q = qo;
   c.out.ctr = x.d_ctr;
@@ -2802,7 +2818,9 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
         uint32_t grid = t.grid;
         if (!grid) grid = h->cu_count * resident_blocks(t.block, tile_lds(t, cap));
         // queue b: binned[start[b], start[b] + cnt[b]), head queue_head[b]
-        e = launch_tile(t.block, b == 0 ? c0 : c, cap, binned, bw + 16 + b, 0, bw + b,
+        Ctx cb = b == 0 ? c0 : c;
+        cb.kid = (uint32_t)b;
+        e = launch_tile(t.block, cb, cap, binned, bw + 16 + b, 0, bw + b,
                         &x.d_ctr->queue_head[b], grid, s);
         if (e != hipSuccess) return hip_error(h, e, "k_classify batched");
         if (timed) {
@@ -2823,6 +2841,7 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
     for (int t = 0; t < 2; t++) {
       const Caps tc = tile_caps(lds_tiers[t], a.ix.s, h->lds_per_block);
       Ctx ct = c;
+      ct.kid = kTierRow0 + (uint32_t)t;
       ct.out.overflow = list[(t + 1) & 1];
       ct.out.overflow_cnt = &d->tier_n[t + 1];
       const uint32_t grid =
@@ -2830,17 +2849,22 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
       e = launch_tile(lds_tiers[t].block, ct, tc, list[t & 1], nullptr, 0, &d->tier_n[t],
                       &d->tier_head[t], grid, s);
       if (e != hipSuccess) return hip_error(h, e, "k_classify overflow tier");
+      if (timed) hipEventRecord(x.ev[13 + t], s);
     }
     {
       const Caps ca = hbm_caps(h->max_read_len, a.ix.s, kHbmEcap);
       Ctx ch = c;
+      ch.kid = kTierRow0 + 2;
       ch.out.overflow = list[1];
       ch.out.overflow_cnt = &d->tier_n[3];
       hipLaunchKernelGGL(k_classify_hbm<256>, dim3(hbm_blocks), dim3(256), 0, s, ch, ca, big_ws,
                          ws_stride(ca), (const uint32_t *)list[0],
                          (const unsigned long long *)&d->tier_n[2], &d->tier_head[2], 0);
+      if (timed) hipEventRecord(x.ev[15], s);
       const Caps cb = hbm_caps(h->max_read_len, a.ix.s, kBigEcap);
-      hipLaunchKernelGGL(k_classify_hbm<1024>, dim3(big_blocks), dim3(1024), 0, s, c, cb, big_ws,
+      Ctx cl = c;
+      cl.kid = kTierRow0 + 3;
+      hipLaunchKernelGGL(k_classify_hbm<1024>, dim3(big_blocks), dim3(1024), 0, s, cl, cb, big_ws,
                          ws_stride(cb), (const uint32_t *)list[1],
                          (const unsigned long long *)&d->tier_n[3], &d->tier_head[3], 1);
     }
@@ -2864,6 +2888,21 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
     if (e != hipSuccess) return hip_error(h, e, "classify launch");
   }
   return MH_OK;
+}
+
+void kernel_counters_from(const DevCounters &d, mh_counters *o, uint32_t n) {
+  for (uint32_t k = 0; k < n && k < (uint32_t)kKernelRows; k++) {
+    const unsigned long long *t = d.kstats[k];
+    o[k].queries = t[kStrands];
+    o[k].probes = t[kProbes];
+    o[k].hits = t[kHits];
+    o[k].entries = t[kEntries];
+    o[k].seeds = t[kSeeds];
+    o[k].cands = t[kCands];
+    o[k].direct = t[kDirect];
+    o[k].overflow = k >= (uint32_t)kTierRow0 ? d.tier_n[k - kTierRow0] : 0;
+    o[k].ext_compares = t[kExt];
+  }
 }
 
 void counters_from(const DevCounters &d, uint64_t n_reads, mh_counters &o) {

@@ -71,7 +71,11 @@ struct QueryOut {
 
 // Work statistics, accumulated per workgroup and added to one of
 // kStatSlots slots (spread over cache lines) when the workgroup ends.
-enum Stat { kProbes = 0, kHits, kEntries, kSeeds, kCands, kDirect, kExt, kNumStats };
+enum Stat { kProbes = 0, kHits, kEntries, kSeeds, kCands, kDirect, kExt, kStrands, kNumStats };
+// Per-kernel statistics rows: length tiles 0..7, then the overflow tiers
+// (second chance, middle, HBM workspaces, last resort).
+constexpr int kKernelRows = 12;
+constexpr int kTierRow0 = 8;
 constexpr int kStatSlots = 256;
 
 // Device counters, zeroed before each classify call.
@@ -84,6 +88,7 @@ struct DevCounters {
   unsigned long long tier_head[4];  // queue heads of the overflow tiers
   unsigned long long queue_head[8]; // queue heads of the length buckets
   unsigned long long stats[kStatSlots][8];
+  unsigned long long kstats[kKernelRows][8];  // the same, per kernel (kid)
 };
 
 struct Outputs {
@@ -162,8 +167,8 @@ struct Exec {
   DevCounters *h_ctr = nullptr;  // pinned host copy
   // timing events: [0] start, [1..8] after bucket kernels, [9] before the
   // bucket kernels, [10] after the large kernel, [11] after the merge
-  hipEvent_t ev[13] = {};  // 0 start, 1-8 buckets, 9 buckets begin, 10 tiers end,
-                           // 11 end, 12 probe pre-pass begin
+  hipEvent_t ev[17] = {};  // 0 start, 1-8 buckets, 9 buckets begin, 10 tiers end,
+                           // 11 end, 12 probe pre-pass begin, 13-16 after tiers 0-3
   bool timed_probe = false;
   uint32_t tie_factor = 4;       // tie pool entries per query, grown on overflow
   // captured batched call (MH_CLASSIFY_GRAPH) and the arguments it was made for
@@ -251,6 +256,8 @@ size_t classify_scratch_bytes(const mh_handle *h, const IndexView &ix, uint32_t 
                               uint32_t tie_factor, uint32_t flags);
 // Reads the counters of the last call on x (x's stream must be idle).
 void counters_from(const DevCounters &d, uint64_t n_reads, mh_counters &out);
+// The per-kernel rows of the last call on x (x's stream must be idle).
+void kernel_counters_from(const DevCounters &d, mh_counters *out, uint32_t n);
 mh_status launch_score_seeds(mh_handle *h, const IndexView &ix, const Luts &luts,
                              const uint8_t *d_q, const uint64_t *d_qoff, const uint32_t *d_pq,
                              const uint32_t *d_pr, const uint64_t *d_soff, const uint32_t *d_seeds,

@@ -72,7 +72,8 @@ class Counters(ctypes.Structure):
 class Timing(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_float), ("bucket_ms", ctypes.c_float * 8),
                 ("big_ms", ctypes.c_float), ("merge_ms", ctypes.c_float),
-                ("n_buckets", ctypes.c_uint32), ("probe_ms", ctypes.c_float)]
+                ("n_buckets", ctypes.c_uint32), ("probe_ms", ctypes.c_float),
+                ("tier_ms", ctypes.c_float * 4)]
 
 
 class Tile(ctypes.Structure):
@@ -96,11 +97,11 @@ class RequestInfo(ctypes.Structure):
 EXPORTS = ("mh_open", "mh_close", "mh_last_error", "mh_stream", "mh_max_read_len",
            "mh_max_batch_reads", "mh_index_build", "mh_index_free", "mh_index_get_info",
            "mh_index_bucket", "mh_classify", "mh_classify_host", "mh_classify_noalloc",
-           "mh_get_counters", "mh_get_timing", "mh_score_seeds", "mh_set_tiles", "mh_get_tiles",
-           "mh_set_overflow_tiles", "mh_autotune", "mh_server_start",
-           "mh_server_submit", "mh_server_wait", "mh_server_stop", "mh_mhm_open", "mh_mhm_close",
-           "mh_mhm_get_info", "mh_mhm_references", "mh_mhm_bucket", "mh_index_from_mhm",
-           "mh_index_save_mhm", "mh_write_results", "mh_fasta_parse")
+           "mh_get_counters", "mh_get_kernel_counters", "mh_get_timing", "mh_score_seeds",
+           "mh_set_tiles", "mh_get_tiles", "mh_set_overflow_tiles", "mh_autotune",
+           "mh_server_start", "mh_server_submit", "mh_server_wait", "mh_server_stop",
+           "mh_mhm_open", "mh_mhm_close", "mh_mhm_get_info", "mh_mhm_references", "mh_mhm_bucket",
+           "mh_index_from_mhm", "mh_index_save_mhm", "mh_write_results", "mh_fasta_parse")
 
 _lib = None
 
@@ -137,6 +138,7 @@ def load():
         "mh_classify_noalloc": (i32, [P, P, P, P, u32, u32, P, P, u64, P, u64, P, P]),
         "mh_get_counters": (i32, [P, ctypes.POINTER(Counters)]),
         "mh_get_timing": (i32, [P, ctypes.POINTER(Timing)]),
+        "mh_get_kernel_counters": (i32, [P, P, u32, ctypes.POINTER(u32)]),
         "mh_score_seeds": (i32, [P, P, P, P, P, P, P, P, u32, u32, P, P, P, P, P]),
         "mh_set_tiles": (i32, [P, P, u32]),
         "mh_get_tiles": (i32, [P, P, P, u32, ctypes.POINTER(u32), P]),
@@ -313,7 +315,8 @@ class Handle:
         t = Timing()
         self.check(self.lib.mh_get_timing(self.h, ctypes.byref(t)), "mh_get_timing")
         return {"total_ms": t.total_ms, "bucket_ms": list(t.bucket_ms), "big_ms": t.big_ms,
-                "merge_ms": t.merge_ms, "n_buckets": t.n_buckets, "probe_ms": t.probe_ms}
+                "merge_ms": t.merge_ms, "n_buckets": t.n_buckets, "probe_ms": t.probe_ms,
+                "tier_ms": list(t.tier_ms)}
 
     def autotune(self, index, d_reads, d_read_off, n_reads, iters=3):
         """Times tile candidates per length bucket on device-resident reads and
@@ -321,6 +324,15 @@ class Handle:
         self.check(self.lib.mh_autotune(self.h, index.ix, _dev_ptr(d_reads), _dev_ptr(d_read_off),
                                         n_reads, iters), "mh_autotune")
         return self.get_tiles(index)[0]
+
+    def kernel_counters(self):
+        """Per-kernel counters of the last call: rows 0-7 length tiles,
+        8-11 the overflow tiers (second chance, middle, HBM, last resort)."""
+        arr = (Counters * 12)()
+        n = ctypes.c_uint32()
+        self.check(self.lib.mh_get_kernel_counters(self.h, ctypes.cast(arr, ctypes.c_void_p), 12,
+                                                   ctypes.byref(n)), "mh_get_kernel_counters")
+        return [{f: getattr(arr[i], f) for f in COUNTER_FIELDS} for i in range(n.value)]
 
     def counters(self):
         c = Counters()

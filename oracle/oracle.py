@@ -232,5 +232,9 @@ def std_stable_sort_tax(ids, taxids):
 
 
 def top_sets(out, top):
-    """Per-read sorted tie sets as tuples."""
+    """Per-read sorted tie sets as tuples.  The top array must have held
+    every tie set (classify's top_cap at least the largest n_top)."""
+    if len(out) and int(out["n_top"].max()) > top.shape[1]:
+        raise ValueError(f"a tie set of {int(out['n_top'].max())} references exceeds top_cap "
+                         f"{top.shape[1]}: classify again with a larger top_cap")
     return [tuple(top[i, :out["n_top"][i]].tolist()) for i in range(len(out))]

@@ -234,3 +234,27 @@ def test_workgroup_teams_score_many_references(handle, flags, table):
     compare_cands(cands, ocands, f"workgroup teams {table}")
     per_strand = np.unique(cands[["read", "strand"]], return_counts=True)[1]
     assert per_strand.max() > 32  # three or more batches of slots
+
+
+@pytest.mark.parametrize("flags", [0, native.MH_CLASSIFY_BATCHED])
+def test_kernel_counters_add_up(handle, flags):
+    """Every strand finishes in exactly one kernel: the per-kernel rows'
+    strands add up to both strands of every read, and their work fields to
+    the call's totals; strands reach the HBM tiers here."""
+    rb, ro, seg = shared_segment(300, 2400, 500, seed=11)
+    rng = np.random.default_rng(13)
+    reads = [seg[a:a + ln] for a, ln in ((0, 1500), (300, 2000), (100, 150), (900, 150))]
+    for i in range(40):
+        a = int(ro[i * 7]) + int(rng.integers(0, 300))
+        reads.append(rb[a:a + int(rng.integers(100, 400))])
+    qb, qo = pack(reads)
+    ix = handle.build_index(rb, ro, 30)
+    ix.classify_host(qb, qo, flags=flags)
+    tot = handle.counters()
+    rows = handle.kernel_counters()
+    assert len(rows) == 12
+    assert sum(r["queries"] for r in rows) == 2 * (len(qo) - 1)
+    for f in ("probes", "hits", "entries", "seeds", "cands", "direct", "ext_compares"):
+        assert sum(r[f] for r in rows) == tot[f], f
+    assert rows[10]["overflow"] > 0 and rows[10]["queries"] + rows[11]["queries"] > 0
+    ix.free()
