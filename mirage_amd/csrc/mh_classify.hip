@@ -171,7 +171,7 @@ struct BlockTeam {
   static constexpr uint32_t kSize = BS;
   static constexpr uint32_t kWorkerLanes = 64;
   static constexpr uint32_t kPerWave = 1;        // teams per workgroup
-  static constexpr uint32_t kSeedLanes = 16;     // lanes that extend one seed
+  static constexpr uint32_t kSeedLanes = 8;      // lanes that extend one seed
   __device__ static uint32_t rank() { return threadIdx.x; }
   __device__ static uint32_t index() { return 0; }
   __device__ static uint32_t worker() { return threadIdx.x >> 6; }
@@ -560,8 +560,11 @@ __device__ __forceinline__ float matrix_score(uint64_t sum_r, uint64_t sum_c, ui
 // Head/tail extension of one seed by one seed worker (MatrixCPU.cpp:93-172).
 // Returns the matched length tot and the head length h; all lanes agree.
 // Tail and head are independent, so each round loads a chunk of both before
-// the ballots find the first mismatch: one memory round trip covers
-// kSeedLanes bases in each direction.
+// the ballots find the first mismatch: every lane loads 64 / kSeedLanes
+// consecutive bases in each direction, so one memory round trip covers 64
+// bases (the whole extension at T <= 64).  The first mismatch is the lowest
+// position of the first lane whose bases hold one; that lane's position is
+// read with a cross-lane permute.
 template <class TM>
 __device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const uint8_t *rb,
                                             uint64_t ref_len, uint32_t sref, uint32_t rs,
@@ -592,26 +595,42 @@ __device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const 
     t = tl;
     h = lb;
     bool tdone = tl == 0, hdone = hl == 0;
-    for (uint32_t b = 0; !(tdone && hdone); b += W) {
-      const uint32_t l = b + lane;
-      const bool vt = !tdone && l < tl, vh = !hdone && l < hl;
-      const uint8_t rt = vt ? rb[(uint64_t)lref + l] : 0;  // both loads issue first
-      const uint8_t rh = vh ? rb[sref - l] : 0;
-      const unsigned long long bt = seed_ballot<TM>(vt && q[re + l] != rt);
-      const unsigned long long bh = seed_ballot<TM>(vh && q[rs - 1 - l] != rh);
+    constexpr uint32_t BL = 64 / W;  // bases per lane, direction and round
+    constexpr uint32_t CH = BL * W;
+    for (uint32_t b = 0; !(tdone && hdone); b += CH) {
+      const uint32_t l0 = b + BL * lane;
+      uint8_t rt[BL], rh[BL];
+#pragma unroll
+      for (uint32_t j = 0; j < BL; j++) {  // every load issues before any compare
+        const uint32_t l = l0 + j;
+        rt[j] = (!tdone && l < tl) ? rb[(uint64_t)lref + l] : 0;
+        rh[j] = (!hdone && l < hl) ? rb[sref - l] : 0;
+      }
+      uint32_t tm = 0, hm = 0;  // bit j: base l0 + j differs
+#pragma unroll
+      for (uint32_t j = 0; j < BL; j++) {
+        const uint32_t l = l0 + j;
+        if (!tdone && l < tl && q[re + l] != rt[j]) tm |= 1u << j;
+        if (!hdone && l < hl && q[rs - 1 - l] != rh[j]) hm |= 1u << j;
+      }
+      const unsigned long long bt = seed_ballot<TM>(tm != 0);
+      const unsigned long long bh = seed_ballot<TM>(hm != 0);
+      // each lane's first differing position (read from the first lane that has one)
+      const uint32_t pt = l0 + (tm ? (uint32_t)__ffs(tm) - 1 : 0u);
+      const uint32_t ph = l0 + (hm ? (uint32_t)__ffs(hm) - 1 : 0u);
       if (!tdone) {
         if (bt) {
-          t = b + (uint32_t)__ffsll((long long)bt) - 1;
+          t = bperm(src_at<W>((uint32_t)__ffsll((long long)bt) - 1), pt);
           tdone = true;
-        } else if (b + W >= tl) {
+        } else if (b + CH >= tl) {
           tdone = true;
         }
       }
       if (!hdone) {
         if (bh) {
-          h = b + (uint32_t)__ffsll((long long)bh) - 1;
+          h = bperm(src_at<W>((uint32_t)__ffsll((long long)bh) - 1), ph);
           hdone = true;
-        } else if (b + W >= hl) {
+        } else if (b + CH >= hl) {
           hdone = true;
         }
       }
@@ -754,8 +773,12 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
       const uint32_t nitems = ws.red[kBcast + 2];
       unsigned long long ncmp = 0;
       for (uint32_t i = wid; i < nitems; i += NW) {
-        uint32_t b = 0;  // the slot of item i
-        while (b + 1 < nb && ws.slots[b + 1].base <= i) b++;
+        uint32_t b = 0, hi = nb;  // the slot of item i: the last with base <= i
+        while (hi - b > 1) {
+          const uint32_t mid = (b + hi) >> 1;
+          if (ws.slots[mid].base <= i) b = mid;
+          else hi = mid;
+        }
         const RefSlot &sl = ws.slots[b];
         const uint32_t jj = i - sl.base;
         uint32_t sref, sq, lref, lq, wt;
@@ -800,14 +823,13 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
       }
       if (lane == 0 && ncmp && ws.stats) atomicAdd(&ws.stats[kExt], ncmp);
       TM::sync();
-      // one wavefront per slot: row hits, Score, relative score
-      const uint32_t wave = tid >> 6, l64 = tid & 63;
-      for (uint32_t b = wave; b < nb; b += BS / 64) {
+      // one thread per slot: row hits, Score, relative score (the table
+      // lookups of every slot in flight together)
+      for (uint32_t b = tid; b < nb; b += BS) {
         const uint32_t *bm = ws.slot_bm + b * bw;
-        uint32_t pc = 0;
-        for (uint32_t w = l64; w < nw; w += 64) pc += __popc(bm[w]);
-        const uint32_t hits = wave_sum(pc);
-        if (l64 == 0) {
+        uint32_t hits = 0;
+        for (uint32_t w = 0; w < nw; w++) hits += __popc(bm[w]);
+        {
           const RefSlot &sl = ws.slots[b];
           const float score = matrix_score(sl.acc[0], sl.acc[1], hits, m);
           float rel = -1.0f;
@@ -1013,15 +1035,36 @@ __device__ void filter_hashed(const Ws &ws, uint32_t E, uint32_t ecap, uint32_t 
 // hit sets one bit, Analyser.cpp:2057-2058); its longest run of set bits is
 // its longest run of consecutive hit indices.  Same decisions as
 // filter_hashed without its sort.  Kept pairs go to k2 packed as
-// ref << 38 | refPos << 6 | hit, unsorted.  Needs next_pow2(2E) <= ecap (the
-// masks take t1..t2, the keys t3, the runs rref) and references < 2^26.
-template <class TM>
-__device__ void filter_masked(const Ws &ws, uint32_t E, uint32_t &nP) {
+// ref << 38 | refPos << 6 | hit, unsorted.  The map has `cap` slots, a power
+// of two above E (every reference then finds one), in the t1..rref region:
+// masks (MT: 32 bits for up to 32 hits, else 64), then the keys; runs are
+// recomputed from the masks.  References < 2^26.
+__device__ __forceinline__ uint32_t longest_run(uint64_t x) {
+  uint32_t r = 0;
+  while (x) {
+    x &= x >> 1;
+    r++;
+  }
+  return r;
+}
+
+// Map slots filter_masked can use: twice the entries when the region has
+// room, else the largest power of two it holds; 0 when that is not above E.
+__device__ __forceinline__ uint32_t masked_slots(uint32_t E, uint32_t nhits, uint32_t ecap) {
+  const uint32_t words = 4 * ecap;                       // t1..rref
+  const uint32_t per_slot = nhits <= 32 ? 2u : 3u;       // mask words + key
+  const uint32_t room = 1u << (31 - __clz(words / per_slot));
+  const uint32_t cap = min(next_pow2(2 * E), room);
+  return cap > E ? cap : 0u;
+}
+
+template <class TM, class MT>
+__device__ void filter_masked(const Ws &ws, uint32_t E, uint32_t cap, uint32_t &nP) {
   constexpr uint32_t BS = TM::kSize;
   const uint32_t tid = TM::rank();
-  const uint32_t cap = next_pow2(2 * E), mask = cap - 1;
-  unsigned long long *hm = reinterpret_cast<unsigned long long *>(ws.t1);
-  uint32_t *rkeys = ws.t3, *run = ws.rref;
+  const uint32_t mask = cap - 1;
+  MT *hm = reinterpret_cast<MT *>(ws.t1);
+  uint32_t *rkeys = ws.t1 + cap * (uint32_t)(sizeof(MT) / 4);
   for (uint32_t i = tid; i < cap; i += BS) {
     rkeys[i] = kEmpty;
     hm[i] = 0;
@@ -1030,21 +1073,12 @@ __device__ void filter_masked(const Ws &ws, uint32_t E, uint32_t &nP) {
   for (uint32_t e = tid; e < E; e += BS) {
     const uint64_t k = ws.k1[e];
     const uint32_t slot = map_slot(rkeys, mask, (uint32_t)(k >> 32));
-    atomicOr(&hm[slot], 1ull << ((uint32_t)k & 63));
+    atomicOr(&hm[slot], (MT)1 << ((uint32_t)k & 63));
     ws.v2[e] = slot;
   }
   TM::sync();
   uint32_t mr = 0;
-  for (uint32_t i = tid; i < cap; i += BS) {
-    uint64_t x = hm[i];
-    uint32_t r = 0;
-    while (x) {
-      x &= x >> 1;
-      r++;
-    }
-    run[i] = r;
-    mr = max(mr, r);
-  }
+  for (uint32_t i = tid; i < cap; i += BS) mr = max(mr, longest_run(hm[i]));
   const uint32_t maxrun = TM::max(mr, ws.red);
   const uint32_t cut = maxrun >= 2 ? maxrun - 1 : 1;
   uint32_t nkeep = 0;
@@ -1054,7 +1088,7 @@ __device__ void filter_masked(const Ws &ws, uint32_t E, uint32_t &nP) {
     uint64_t pk = 0;
     if (e < E) {
       const uint64_t k = ws.k1[e];
-      keep = run[ws.v2[e]] >= cut ? 1u : 0u;
+      keep = longest_run(hm[ws.v2[e]]) >= cut ? 1u : 0u;
       pk = ((k >> 32) << 38) | ((uint64_t)ws.v1[e] << 6) | ((uint32_t)k & 63);
     }
     uint32_t tot;
@@ -1642,8 +1676,10 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
     // pairs in k2/v2 as (ref << 32 | refPos, qPos), unsorted.
     // (Done in registers by small_path for small working sets.)
     if (!sorted) {
-      if (nhits <= 64 && ix.n_refs <= (1u << 26) && next_pow2(2 * E) <= cap.ecap) {
-        filter_masked<TM>(ws, E, nP);
+      const uint32_t mslots = nhits <= 64 ? masked_slots(E, nhits, cap.ecap) : 0u;
+      if (mslots && ix.n_refs <= (1u << 26)) {
+        if (nhits <= 32) filter_masked<TM, uint32_t>(ws, E, mslots, nP);
+        else filter_masked<TM, unsigned long long>(ws, E, mslots, nP);
         // Sort by (reference, refPos, qPos) (Analyser.cpp:1024-1059): the
         // packed keys order so (hit order is qPos order); then unpack.
         if (ws.stage)
@@ -2479,10 +2515,22 @@ __global__ void k_max_len(Reads rd, unsigned long long *out) {
 // them at any length (about 66 KB each for 32 kb reads).
 constexpr uint32_t kRefSlots = 16;
 constexpr uint32_t kRefSlotMaxLen = 2048;
+// LDS tiles give the slots about 6 KB: short reads (small row bitmaps) get
+// more of them, up to 64 (a strand of a segment shared by 120 references
+// then scores them in two rounds instead of eight).
+constexpr size_t kSlotBytes = 6144;
+constexpr uint32_t kMaxRefSlots = 64;
+
+uint32_t lds_slots(uint32_t qcap) {
+  const size_t per = sizeof(RefSlot) + (qcap / 32 + 1) * 4ull;
+  return (uint32_t)std::min<size_t>(kMaxRefSlots, std::max<size_t>(kRefSlots, kSlotBytes / per));
+}
 
 Caps caps_for(const mh_tile &t, int s, size_t limit) {
   Caps c;
-  c.nslot = (t.block >= 128 && t.max_len <= kRefSlotMaxLen) ? kRefSlots : 0;
+  c.nslot = (t.block >= 128 && t.max_len <= kRefSlotMaxLen)
+                ? lds_slots((t.max_len + 63) & ~63u)
+                : 0;
   const uint32_t sd = (uint32_t)(s > 0 ? s : 1);
   c.qcap = (t.max_len + 63) & ~63u;
   c.pcap = c.qcap / sd + 2;
