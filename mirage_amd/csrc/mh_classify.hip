@@ -1059,7 +1059,7 @@ __device__ __forceinline__ uint32_t masked_slots(uint32_t E, uint32_t nhits, uin
 }
 
 template <class TM, class MT>
-__device__ void filter_masked(const Ws &ws, uint32_t E, uint32_t cap, uint32_t &nP) {
+__device__ uint32_t filter_masked(const Ws &ws, uint32_t E, uint32_t cap, uint32_t &nP) {
   constexpr uint32_t BS = TM::kSize;
   const uint32_t tid = TM::rank();
   const uint32_t mask = cap - 1;
@@ -1077,10 +1077,17 @@ __device__ void filter_masked(const Ws &ws, uint32_t E, uint32_t cap, uint32_t &
     ws.v2[e] = slot;
   }
   TM::sync();
+  // each entry's run, from its reference's mask (v2 takes the run)
   uint32_t mr = 0;
-  for (uint32_t i = tid; i < cap; i += BS) mr = max(mr, longest_run(hm[i]));
+  for (uint32_t e = tid; e < E; e += BS) {
+    const uint32_t r = longest_run(hm[ws.v2[e]]);
+    ws.v2[e] = r;
+    mr = max(mr, r);
+  }
   const uint32_t maxrun = TM::max(mr, ws.red);
-  const uint32_t cut = maxrun >= 2 ? maxrun - 1 : 1;
+  nP = 0;
+  if (maxrun < 2) return maxrun;  // cut 1 keeps every pair, and none links
+  const uint32_t cut = maxrun - 1;
   uint32_t nkeep = 0;
   for (uint32_t base = 0; base < E; base += BS) {
     const uint32_t e = base + tid;
@@ -1088,7 +1095,7 @@ __device__ void filter_masked(const Ws &ws, uint32_t E, uint32_t cap, uint32_t &
     uint64_t pk = 0;
     if (e < E) {
       const uint64_t k = ws.k1[e];
-      keep = longest_run(hm[ws.v2[e]]) >= cut ? 1u : 0u;
+      keep = ws.v2[e] >= cut ? 1u : 0u;
       pk = ((k >> 32) << 38) | ((uint64_t)ws.v1[e] << 6) | ((uint32_t)k & 63);
     }
     uint32_t tot;
@@ -1098,6 +1105,7 @@ __device__ void filter_masked(const Ws &ws, uint32_t E, uint32_t cap, uint32_t &
   }
   nP = nkeep;
   TM::sync();
+  return maxrun;
 }
 
 // Successive-occurrence filter by sorting every pair by (ref, hit); used
@@ -1647,7 +1655,13 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
     }
   }
 
-  if (!direct && !sorted) {
+  // The hit-mask filter (below) needs at most 64 hits and a map of more
+  // slots than entries; it also tells when no reference occurs in two
+  // consecutive hits, which the link pre-check would find at a higher cost.
+  const uint32_t mslots = (!direct && !sorted && nhits <= 64 && ix.n_refs <= (1u << 26))
+                              ? masked_slots(E, nhits, cap.ecap)
+                              : 0u;
+  if (!direct && !sorted && !mslots) {
     // A seed needs two pairs of one reference one stride apart on both axes
     // (a chain link).  Links only join pairs of consecutive probes, so when
     // no entry of a hit has its successor in the next probe's bucket, every
@@ -1676,10 +1690,13 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
     // pairs in k2/v2 as (ref << 32 | refPos, qPos), unsorted.
     // (Done in registers by small_path for small working sets.)
     if (!sorted) {
-      const uint32_t mslots = nhits <= 64 ? masked_slots(E, nhits, cap.ecap) : 0u;
-      if (mslots && ix.n_refs <= (1u << 26)) {
-        if (nhits <= 32) filter_masked<TM, uint32_t>(ws, E, mslots, nP);
-        else filter_masked<TM, unsigned long long>(ws, E, mslots, nP);
+      if (mslots) {
+        const uint32_t maxrun = nhits <= 32 ? filter_masked<TM, uint32_t>(ws, E, mslots, nP)
+                                            : filter_masked<TM, unsigned long long>(ws, E, mslots, nP);
+        if (maxrun < 2) {  // no reference in two consecutive hits: no link, no seeds
+          write_query<TM>(c, q, res);
+          return kDone;
+        }
         // Sort by (reference, refPos, qPos) (Analyser.cpp:1024-1059): the
         // packed keys order so (hit order is qPos order); then unpack.
         if (ws.stage)
