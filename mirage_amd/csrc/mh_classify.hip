@@ -261,6 +261,27 @@ __device__ __forceinline__ bool pair_less(uint64_t ka, uint32_t va, uint64_t kb,
 // i ^ (k - 1), then with i ^ d for d = k/4 .. 1.  Positions n and beyond act
 // as +infinity and never move, so the comparisons that would touch them are
 // skipped and nothing is padded.
+// Synchronisation before a network step.  Thread t of a team handles pairs
+// p = t, t + kSize, ...; a wavefront's pairs P..P+63 (P a multiple of 64)
+// touch elements 2P..2P+127 in a step of stride up to 64, and in a mirror
+// step of size up to 128, so between two such steps each wavefront only
+// waits for itself (its LDS accesses execute in order); any other step, and
+// the step after it, needs the whole team.
+__device__ __forceinline__ bool wave_local_step(uint32_t size, uint32_t stride) {
+  return stride == (size >> 1) ? size <= 128 : stride <= 64;
+}
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <class TM>
+__device__ __forceinline__ void step_sync(bool local, bool &prev_local) {
+  if (local && prev_local) wave_sync();
+  else TM::sync();
+  prev_local = local;
+}
+
 template <class TM>
 __device__ void bitonic_sort(uint64_t *key, uint32_t *val, uint32_t n) {
   constexpr uint32_t W = TM::kWorkerLanes;
@@ -287,9 +308,10 @@ __device__ void bitonic_sort(uint64_t *key, uint32_t *val, uint32_t n) {
     return;
   }
   const uint32_t n2 = next_pow2(n);
+  bool prev_local = false;
   for (uint32_t size = 2; size <= n2; size <<= 1) {
     for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-      TM::sync();
+      step_sync<TM>(wave_local_step(size, stride), prev_local);
       const uint32_t lg = 31 - __clz(stride);
       const bool mirror = stride == (size >> 1);
       for (uint32_t t = TM::rank(); t < (n2 >> 1); t += TM::kSize) {
@@ -331,9 +353,10 @@ __device__ void bitonic_sort_keys(uint64_t *key, uint32_t n) {
     return;
   }
   const uint32_t n2 = next_pow2(n);
+  bool prev_local = false;
   for (uint32_t size = 2; size <= n2; size <<= 1) {
     for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-      TM::sync();
+      step_sync<TM>(wave_local_step(size, stride), prev_local);
       const uint32_t lg = 31 - __clz(stride);
       const bool mirror = stride == (size >> 1);
       for (uint32_t t = TM::rank(); t < (n2 >> 1); t += TM::kSize) {
@@ -708,7 +731,8 @@ __device__ void score_ref(const uint8_t *q, uint32_t m, const uint8_t *rb, uint6
 // memory round trips of different references overlap.  Per reference the
 // sums, bitmap and score are those of score_ref (integer sums and a bitwise
 // OR do not depend on the order).  Sets rel_of / valid_of per rank (0 for
-// ranks not scored); seeds_total, seeded and lut_bad are meaningful on rank 0.
+// ranks not scored); seeds_total (rank 0's share; the other threads add
+// theirs to the statistics), seeded and lut_bad are meaningful on rank 0.
 template <class TM>
 __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Caps &cap,
                                             uint32_t m, uint32_t nranks, uint32_t nchains,
@@ -727,6 +751,7 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
   const uint32_t bw = cap.qcap / 32 + 1, nw = (m + 31) >> 5;
   seeds_total = 0;
   seeded = 0;
+  unsigned long long seeds_part = 0;  // this thread's slots' seeds
   for (uint32_t r = tid; r < nranks; r += BS) valid_of[r] = 0;
   if (tid == 0) ws.red[kBcast + 3] = 0;  // a relative score outside the tables
   for (uint32_t base = 0; base < nranks; base += BS) {
@@ -758,19 +783,20 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
       }
       for (uint32_t w = tid; w < nb * bw; w += BS) ws.slot_bm[w] = 0;
       TM::sync();
-      if (tid == 0) {
-        uint32_t acc = 0;
-        for (uint32_t b = 0; b < nb; b++) {
-          ws.slots[b].base = acc;
-          acc += ws.slots[b].nseed;
-          seeds_total += ws.slots[b].nchain +
-                         (unsigned long long)(ws.slots[b].nseed - ws.slots[b].nchain) * ix.phantom;
+      // each slot's first item: a scan of the seed counts (nb <= BS)
+      uint32_t nitems;
+      {
+        const uint32_t ns = tid < nb ? ws.slots[tid].nseed : 0u;
+        const uint32_t base_t = TM::excl_scan(ns, ws.red, nitems);
+        if (tid < nb) {
+          ws.slots[tid].base = base_t;
+          seeds_part += ws.slots[tid].nchain +
+                        (unsigned long long)(ws.slots[tid].nseed - ws.slots[tid].nchain) *
+                            ix.phantom;
         }
         seeded += nb;
-        ws.red[kBcast + 2] = acc;
       }
       TM::sync();
-      const uint32_t nitems = ws.red[kBcast + 2];
       unsigned long long ncmp = 0;
       for (uint32_t i = wid; i < nitems; i += NW) {
         uint32_t b = 0, hi = nb;  // the slot of item i: the last with base <= i
@@ -862,6 +888,9 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
       }
     }
   }
+  // rank 0 adds the count of seeds to the statistics with the others'
+  if (seeds_part && ws.stats && tid != 0) atomicAdd(&ws.stats[kSeeds], seeds_part);
+  if (tid == 0) seeds_total = seeds_part;
   TM::sync();
   lut_bad = ws.red[kBcast + 3];
 }
@@ -1546,16 +1575,36 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
   }
 
   // Gather bucket entries: k1 = ref << 32 | hit index, v1 = ref position.
-  for (uint32_t e = tid; e < E; e += BS) {
-    uint32_t lo = 0, hi = nhits;  // largest i with hit_base[i] <= e
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (ws.hit_base[mid] <= e) lo = mid;
-      else hi = mid;
+  // Four entries per thread at a time, their loads issued together: one
+  // memory round trip for up to 4 * BS entries.
+  constexpr uint32_t kGather = 4;
+  for (uint32_t e0 = tid; e0 < E; e0 += kGather * BS) {
+    uint2 en[kGather];
+    uint32_t hit[kGather];
+#pragma unroll
+    for (uint32_t u = 0; u < kGather; u++) {
+      const uint32_t e = e0 + u * BS;
+      hit[u] = 0;
+      en[u] = make_uint2(0, 0);
+      if (e < E) {
+        uint32_t lo = 0, hi = nhits;  // largest i with hit_base[i] <= e
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (ws.hit_base[mid] <= e) lo = mid;
+          else hi = mid;
+        }
+        hit[u] = lo;
+        en[u] = ix.ent[ws.hit_lo[lo] + (e - ws.hit_base[lo])];
+      }
     }
-    const uint2 en = ix.ent[ws.hit_lo[lo] + (e - ws.hit_base[lo])];
-    ws.k1[e] = ((uint64_t)en.x << 32) | lo;
-    ws.v1[e] = en.y;
+#pragma unroll
+    for (uint32_t u = 0; u < kGather; u++) {
+      const uint32_t e = e0 + u * BS;
+      if (e < E) {
+        ws.k1[e] = ((uint64_t)en[u].x << 32) | hit[u];
+        ws.v1[e] = en[u].y;
+      }
+    }
   }
   TM::sync();
 
