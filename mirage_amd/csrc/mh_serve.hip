@@ -5,11 +5,11 @@
 // context (its own stream, scratch arena and counters) with device and
 // pinned host buffers for one coalesced batch.  Whenever a slot is free, the
 // scheduler packs the oldest pending requests - as many as fit the slot's
-// read and byte capacity and the remaining HBM budget - into one batch,
+// read and byte capacity and the remaining HBM budget - into one batch and
+// hands it to a thread of the slot, which stages it into pinned memory,
 // copies it up, launches the batched classification and copies the results
-// back, all asynchronously on the slot's stream.  A completion thread per
-// slot waits for the stream, splits the results per request and wakes the
-// waiters.  Requests that arrive while batches are running are admitted as
+// back on the slot's stream, then waits for the stream, splits the results
+// per request and wakes the waiters.  Requests that arrive while batches are running are admitted as
 // soon as a slot and budget free up, so batches overlap on the device.
 // Per-read results do not depend on batch composition: every query is
 // processed independently and the tie pool is compacted per batch.
@@ -119,7 +119,8 @@ mh_status init_slot(mh_handle *h, Slot &s, uint64_t cap_bytes, uint64_t cap_read
                                                         flags));
 }
 
-// Launch the slot's batch (called with the server lock held).
+// Launch the slot's batch, from the slot's thread without the server lock:
+// the slot is busy and its batch's requests are no longer pending.
 mh_status launch_slot(mh_server *sv, Slot &s) {
   mh_handle *h = sv->h;
   uint64_t pos = 0;
@@ -270,8 +271,16 @@ void scheduler(mh_server *sv) {
       }
       if (s.batch.empty()) continue;
       s.busy = true;
-      s.launched_ok = launch_slot(sv, s) == MH_OK;
-      s.waiter = std::thread(complete_slot, sv, &s);
+      // The batch belongs to the slot now: the slot's own thread stages it
+      // (host copies into pinned memory), launches it and completes it, so
+      // the batches of several slots are staged in parallel and the
+      // scheduler goes on admitting meanwhile.
+      Slot *sp_ = &s;
+      s.waiter = std::thread([sv, sp_] {
+        hipSetDevice(sv->h->device);
+        sp_->launched_ok = launch_slot(sv, *sp_) == MH_OK;
+        complete_slot(sv, sp_);
+      });
       launched = true;
     }
     if (!launched) sv->cv_sched.wait_for(lk, std::chrono::milliseconds(50));

@@ -623,6 +623,31 @@ class Server:
         self.sizes[rid.value] = len(read_off) - 1
         return rid.value
 
+    def submit_arrays(self, reads, read_off):
+        """submit() without copies or checks: reads (uint8) and read_off
+        (uint64) must be contiguous numpy arrays; the library copies them."""
+        rid = ctypes.c_uint64()
+        n = len(read_off) - 1
+        st = self.lib.mh_server_submit(self.s, reads.ctypes.data, read_off.ctypes.data, n,
+                                       ctypes.byref(rid))
+        if st != MH_OK:
+            raise MirageError(st, "mh_server_submit")
+        self.sizes[rid.value] = n
+        return rid.value
+
+    def wait_into(self, rid, out, top, timeout=-1.0):
+        """wait() into caller buffers (out: READ_RESULT_DTYPE of at least the
+        request's reads; top: uint32).  Returns (top ids needed, info)."""
+        n = self.sizes.pop(rid)
+        need = ctypes.c_uint64()
+        info = RequestInfo()
+        st = self.lib.mh_server_wait(self.s, rid, timeout, out.ctypes.data, top.ctypes.data,
+                                     len(top), ctypes.byref(need), ctypes.byref(info))
+        if st != MH_OK:
+            raise MirageError(st, f"request {rid}")
+        return int(need.value), {"submit_s": info.submit_s, "admit_s": info.admit_s,
+                                 "done_s": info.done_s, "n_reads": n}
+
     def wait(self, rid, timeout=-1.0):
         """Returns (results, top pool, info dict) of request rid."""
         n = self.sizes.pop(rid)
