@@ -576,7 +576,7 @@ def workload4_leg(args, torch, native, D):
     h = native.Handle(0, 8192)
     ix = h.build_index(rb, ro, w.threshold)
     rng = np.random.default_rng(w.seed)
-    reqs, sizes = make_requests(rng, qb, qo, args.serve_requests)
+    reqs, sizes = make_requests(rng, qb, qo, args.w4_requests)
     log(f"workload 4: {len(ro) - 1} refs, {len(rb) / 1e9:.2f} Gbp; {len(reqs)} requests")
     sv = serve_stream(native, ix, reqs, sizes, (0.25, 0.5, 0.9), rng, log)
     sv["workload"] = (f"{len(ro) - 1} refs {len(rb) / 1e9:.2f} Gbp k=16; requests of "
@@ -605,7 +605,9 @@ def main():
     ap.add_argument("--w2-cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--max-read-len", type=int, default=4096)
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-requests", type=int, default=4000,
+                    help="requests of the headline's served stream")
+    ap.add_argument("--w4-requests", type=int, default=300)
     ap.add_argument("--plain-requests", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
