@@ -41,11 +41,8 @@ size_t Get_LOCAL_DATA_rowhit_SIZE() { return device(0).MaxReadLength(); }
 
 size_t CUDAGetMaxBlockSize(int cudaDeviceID) { return device(cudaDeviceID).MaxBlockSize(); }
 
-void MirageHipLoadReferences(const vector<vector<char> *> &refs, int32_t considerThreshold) {
-  std::vector<std::vector<char>> copy;
-  copy.reserve(refs.size());
-  for (const vector<char> *r : refs) copy.emplace_back(r->begin(), r->end());
-  for (auto &d : g_dev) d->LoadReferences(copy, considerThreshold);
+void MirageHipLoadReferences(const vector<vector<char>> &refs, int32_t considerThreshold) {
+  for (auto &d : g_dev) d->LoadReferences(refs, considerThreshold);
 }
 
 // Scores the slice [queriesToAnalyse_Offset, +queriesToAnalyse_SizePerAnalysis)

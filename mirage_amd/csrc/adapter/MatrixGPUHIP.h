@@ -24,4 +24,4 @@ size_t CUDAGetMaxBlockSize(int cudaDeviceID);
 // (Sequences::referenceSequences) with the run's considerthreshold.  No
 // counterpart in the reference: its engine re-sends reference bytes with
 // every call, this one keeps them resident in HBM.
-void MirageHipLoadReferences(const vector<vector<char> *> &refs, int32_t considerThreshold);
+void MirageHipLoadReferences(const vector<vector<char>> &refs, int32_t considerThreshold);

@@ -76,8 +76,8 @@ int main() {
   printf("rowhit\t%zu\n", Get_LOCAL_DATA_rowhit_SIZE());
   printf("blocksize\t%zu\n", CUDAGetMaxBlockSize(1));
 
-  vector<char> r0{'A', 'C', 'G'}, r1{'T', 'T'};
-  MirageHipLoadReferences({&r0, &r1}, 31);
+  vector<vector<char>> refs{{'A', 'C', 'G'}, {'T', 'T'}};
+  MirageHipLoadReferences(refs, 31);
   printf("loaded\t%zu %zu %d\n", g_loaded_refs, g_loaded_bases, g_loaded_T);
 
   // Five queries; the call scores the slice of queries 1..3 and the
