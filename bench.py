@@ -617,6 +617,9 @@ def main():
     ap.add_argument("--no-w4", action="store_true")
     ap.add_argument("--no-mhm", action="store_true")
     ap.add_argument("--no-fasta", action="store_true")
+    ap.add_argument("--only-w2", action="store_true",
+                    help="profiling run: workload 2 (and the mixed batch unless --no-mixed) "
+                         "alone; the line's value is workload 2's")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also copy 1 GiB device to device once, the reference kernel for "
                          "scaling HBM byte counters in a profiler run")
@@ -638,6 +641,16 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration: 1 GiB read + 1 GiB written")
         del a, b
+    if args.only_w2:
+        w2, mixed = workload2_leg(args, torch, native, D)
+        print(json.dumps({
+            "metric": "reads classified/sec on 1 MI355X (profiling run, workload 2 only)",
+            "value": w2["reads_per_s"], "unit": "reads/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": w2["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic (seeded)",
+            "config": {"workload": w2["workload"]}, "workload2": w2, "mixed_length": mixed}),
+            flush=True)
+        return
     h5 = headline_leg(args, torch, native, D)
     w2 = mixed = w4 = None
     if not args.no_w2:

@@ -78,27 +78,163 @@ __device__ __forceinline__ uint64_t bperm64(uint32_t src, uint64_t v) {
   return ((uint64_t)bperm(src, (uint32_t)(v >> 32)) << 32) | bperm(src, (uint32_t)v);
 }
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-  const int lane = threadIdx.x & 63;
+// Segment operations on W consecutive lanes (W = 8, 16, 32 or 64, segments
+// aligned to W) through DPP, which moves data between the lanes of a row of
+// 16 inside the VALU, and v_readlane across rows: no LDS round trips.  Every
+// lane of the segment takes part; lanes that have no source read `old`,
+// which is the operation's identity.
+// DPP controls: quad_perm 0x00-0xFF, row_shl:n 0x100+n, row_shr:n 0x110+n,
+// row_ror:n 0x120+n, row_mirror 0x140, row_half_mirror 0x141, row_bcast:15
+// 0x142 (lane 15 of each row to the next row), row_bcast:31 0x143 (lane 31 to
+// rows 2 and 3).
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ uint32_t dpp(uint32_t old, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROWS, 0xF, false);
+}
+__device__ __forceinline__ uint32_t readlane(uint32_t v, uint32_t l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+
+// Inclusive sum over the lanes of the segment up to this one.
+template <uint32_t W>
+__device__ __forceinline__ uint32_t seg_incl_scan(uint32_t v) {
+  static_assert(W == 8 || W == 16 || W == 32 || W == 64, "segment width");
+  const uint32_t p = lane64() & (W - 1);
+  uint32_t t = dpp<0x111>(0u, v);  // row_shr:1
+  v += p >= 1 ? t : 0u;
+  t = dpp<0x112>(0u, v);  // row_shr:2
+  v += p >= 2 ? t : 0u;
+  t = dpp<0x114>(0u, v);  // row_shr:4
+  v += p >= 4 ? t : 0u;
+  if constexpr (W >= 16) {
+    t = dpp<0x118>(0u, v);  // row_shr:8
+    v += p >= 8 ? t : 0u;
+  }
+  if constexpr (W >= 32) v += dpp<0x142, 0xA>(0u, v);  // rows 1, 3 += lane 15 of rows 0, 2
+  if constexpr (W == 64) v += dpp<0x143, 0xC>(0u, v);  // rows 2, 3 += lane 31
+  return v;
+}
+
+// The value of the segment's last lane, on every lane of the segment.
+template <uint32_t W>
+__device__ __forceinline__ uint32_t seg_last(uint32_t v) {
+  if constexpr (W == 64) {
+    return readlane(v, 63);
+  } else {
+    const uint32_t seg = lane64() / W;
+    uint32_t r = 0;
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    uint32_t t = __shfl_up(v, d, 64);
-    if (lane >= d) v += t;
+    for (uint32_t g = 0; g < 64 / W; g++) {
+      const uint32_t x = readlane(v, g * W + W - 1);
+      r = seg == g ? x : r;
+    }
+    return r;
+  }
+}
+
+// Reduction over the segment (op: 0 sum, 1 max, 2 or), on every lane of it.
+template <int OP>
+__device__ __forceinline__ uint32_t lane_op(uint32_t a, uint32_t b) {
+  if constexpr (OP == 0) return a + b;
+  else if constexpr (OP == 1) return a > b ? a : b;
+  else return a | b;
+}
+template <uint32_t W, int OP>
+__device__ __forceinline__ uint32_t seg_reduce(uint32_t v) {
+  static_assert(W == 8 || W == 16 || W == 32 || W == 64, "segment width");
+  if constexpr (W >= 16) v = lane_op<OP>(v, dpp<0x128>(0u, v));  // row_ror:8: lane ^ 8
+  v = lane_op<OP>(v, dpp<0x141>(0u, v));                         // half-row mirror
+  v = lane_op<OP>(v, dpp<0xB1>(0u, v));                          // lane ^ 1
+  v = lane_op<OP>(v, dpp<0x4E>(0u, v));                          // lane ^ 2
+  if constexpr (W == 32) {
+    const uint32_t a = lane_op<OP>(readlane(v, 0), readlane(v, 16));
+    const uint32_t b = lane_op<OP>(readlane(v, 32), readlane(v, 48));
+    v = lane64() < 32 ? a : b;
+  } else if constexpr (W == 64) {
+    v = lane_op<OP>(lane_op<OP>(readlane(v, 0), readlane(v, 16)),
+                    lane_op<OP>(readlane(v, 32), readlane(v, 48)));
   }
   return v;
 }
 
-__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) v = max(v, (uint32_t)__shfl_xor(v, d, 64));
-  return v;
+// The value of lane ^ X (X = 1, 2, 4, 8 inside a row; larger X through
+// ds_bpermute).
+template <uint32_t X>
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v) {
+  if constexpr (X == 1) return dpp<0xB1>(0u, v);
+  else if constexpr (X == 2) return dpp<0x4E>(0u, v);
+  else if constexpr (X == 4) {
+    const uint32_t up = dpp<0x114>(0u, v), down = dpp<0x104>(0u, v);  // lane - 4, lane + 4
+    return (lane64() & 4) ? up : down;
+  } else if constexpr (X == 8) return dpp<0x128>(0u, v);  // row_ror:8
+  else return bperm(src_xor(X), v);
+}
+template <uint32_t X>
+__device__ __forceinline__ uint64_t xor_lane64(uint64_t v) {
+  return ((uint64_t)xor_lane<X>((uint32_t)(v >> 32)) << 32) | xor_lane<X>((uint32_t)v);
 }
 
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) v += (uint32_t)__shfl_xor(v, d, 64);
-  return v;
+// The value of lane - d (lane_up) or lane + d (lane_down) inside segments of
+// W lanes, d a power of two below W; lanes with no such lane read garbage.
+template <uint32_t W>
+__device__ __forceinline__ uint32_t lane_up(uint32_t d, uint32_t v) {
+  if constexpr (W <= 16) {
+    switch (d) {
+      case 1: return dpp<0x111>(0u, v);
+      case 2: return dpp<0x112>(0u, v);
+      case 4: return dpp<0x114>(0u, v);
+      default: return dpp<0x118>(0u, v);
+    }
+  } else {
+    return bperm(src_up<W>(d), v);
+  }
 }
+template <uint32_t W>
+__device__ __forceinline__ uint32_t lane_down(uint32_t d, uint32_t v) {
+  if constexpr (W <= 16) {
+    switch (d) {
+      case 1: return dpp<0x101>(0u, v);
+      case 2: return dpp<0x102>(0u, v);
+      case 4: return dpp<0x104>(0u, v);
+      default: return dpp<0x108>(0u, v);
+    }
+  } else {
+    return bperm(src_down<W>(d), v);
+  }
+}
+template <uint32_t W>
+__device__ __forceinline__ uint64_t lane_up64(uint32_t d, uint64_t v) {
+  return ((uint64_t)lane_up<W>(d, (uint32_t)(v >> 32)) << 32) | lane_up<W>(d, (uint32_t)v);
+}
+template <uint32_t W>
+__device__ __forceinline__ uint64_t lane_down64(uint32_t d, uint64_t v) {
+  return ((uint64_t)lane_down<W>(d, (uint32_t)(v >> 32)) << 32) | lane_down<W>(d, (uint32_t)v);
+}
+__device__ __forceinline__ uint64_t xor_lane64_any(uint32_t x, uint64_t v) {
+  switch (x) {
+    case 1: return xor_lane64<1>(v);
+    case 2: return xor_lane64<2>(v);
+    case 4: return xor_lane64<4>(v);
+    case 8: return xor_lane64<8>(v);
+    default: return bperm64(src_xor(x), v);
+  }
+}
+
+// The value of lane u of the segment (u the same on every lane of the
+// wavefront): v_readlane for whole wavefronts, else ds_bpermute.
+template <uint32_t W>
+__device__ __forceinline__ uint32_t lane_at(uint32_t u, uint32_t v) {
+  if constexpr (W == 64) return readlane(v, u);
+  else return bperm(src_at<W>(u), v);
+}
+template <uint32_t W>
+__device__ __forceinline__ uint64_t lane_at64(uint32_t u, uint64_t v) {
+  return ((uint64_t)lane_at<W>(u, (uint32_t)(v >> 32)) << 32) | lane_at<W>(u, (uint32_t)v);
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) { return seg_incl_scan<64>(v); }
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) { return seg_reduce<64, 1>(v); }
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) { return seg_reduce<64, 0>(v); }
 
 // Exclusive block scan; red holds at least BS/64 words.  All threads get the
 // block total.  Contains barriers when BS > 64.
@@ -107,7 +243,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *red, u
   constexpr int NW = BS / 64;
   const uint32_t inc = wave_incl_scan(v);
   if constexpr (NW == 1) {
-    total = __shfl(inc, 63, 64);
+    total = readlane(inc, 63);
     return inc - v;
   } else {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -211,25 +347,12 @@ struct LaneTeam {
     return wballot(x != 0) != 0;
   }
   __device__ static uint32_t excl_scan(uint32_t v, uint32_t *, uint32_t &tot) {
-    uint32_t inc = v;
-#pragma unroll
-    for (uint32_t d = 1; d < L; d <<= 1) {
-      const uint32_t t = bperm(src_up<L>(d), inc);
-      inc += rank() >= d ? t : 0u;
-    }
-    tot = bperm(src_at<L>(L - 1), inc);
+    const uint32_t inc = seg_incl_scan<L>(v);
+    tot = seg_last<L>(inc);
     return inc - v;
   }
-  __device__ static uint32_t max(uint32_t v, uint32_t *) {
-#pragma unroll
-    for (uint32_t d = L / 2; d > 0; d >>= 1) v = ::max(v, bperm(src_xor(d), v));
-    return v;
-  }
-  __device__ static uint32_t sum(uint32_t v, uint32_t *) {
-#pragma unroll
-    for (uint32_t d = L / 2; d > 0; d >>= 1) v += bperm(src_xor(d), v);
-    return v;
-  }
+  __device__ static uint32_t max(uint32_t v, uint32_t *) { return seg_reduce<L, 1>(v); }
+  __device__ static uint32_t sum(uint32_t v, uint32_t *) { return seg_reduce<L, 0>(v); }
 };
 
 // Seed workers: groups of TM::kSeedLanes lanes of a team, each extending its
@@ -293,9 +416,8 @@ __device__ void bitonic_sort(uint64_t *key, uint32_t *val, uint32_t n) {
       uint32_t v = lane < n ? val[lane] : ~0u;
       uint32_t rank = 0;
       for (uint32_t u = 0; u < n; u++) {
-        const uint32_t src = src_at<W>(u);
-        const uint64_t ku = bperm64(src, k);
-        const uint32_t vu = bperm(src, v);
+        const uint64_t ku = lane_at64<W>(u, k);
+        const uint32_t vu = lane_at<W>(u, v);
         rank += (pair_less(ku, vu, k, v) || (ku == k && vu == v && u < lane)) ? 1u : 0u;
       }
       __builtin_amdgcn_wave_barrier();
@@ -345,7 +467,7 @@ __device__ void bitonic_sort_keys(uint64_t *key, uint32_t n) {
       const uint32_t lane = TM::wlane();
       const uint64_t k = lane < n ? key[lane] : ~0ull;
       uint32_t rank = 0;
-      for (uint32_t u = 0; u < n; u++) rank += bperm64(src_at<W>(u), k) < k ? 1u : 0u;
+      for (uint32_t u = 0; u < n; u++) rank += lane_at64<W>(u, k) < k ? 1u : 0u;
       __builtin_amdgcn_wave_barrier();
       if (lane < n) key[rank] = k;
     }
@@ -1185,10 +1307,9 @@ enum : uint32_t { kSmallNo = 0, kSmallDirect, kSmallSorted };
 
 template <class TM>
 __device__ __forceinline__ uint64_t team_or64(uint64_t x) {
-  constexpr int W = TM::kWorkerLanes;
-#pragma unroll
-  for (uint32_t d = W / 2; d > 0; d >>= 1) x |= bperm64(src_xor(d), x);
-  return x;
+  constexpr uint32_t W = TM::kWorkerLanes;
+  return ((uint64_t)seg_reduce<W, 2>((uint32_t)(x >> 32)) << 32) |
+         seg_reduce<W, 2>((uint32_t)x);
 }
 
 // Small working sets (at most 64 entries and 64 hits, fewer than 2^26
@@ -1296,10 +1417,9 @@ __device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t 
       const uint32_t lx = d >> 1;  // partner lane = lane ^ lx
       // the lower lane of a pair keeps the minimum in an ascending run
       const bool want_min = ((lane & lx) == 0) == asc;
-      const uint32_t src = src_xor(lx);
 #pragma unroll
       for (int j = 0; j < 2; j++) {
-        const uint64_t pk = bperm64(src, P[j]);
+        const uint64_t pk = xor_lane64_any(lx, P[j]);
         P[j] = ((pk < P[j]) == want_min) ? pk : P[j];
       }
     }
@@ -1314,8 +1434,7 @@ __device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t 
     key[j] = P[j] == ~0ull ? ~0ull : ((P[j] >> 38) << 32) | ((P[j] >> 6) & 0xFFFFFFFFull);
   // Segment heads (a new reference), then each reference's hit mask: a
   // segmented OR over its elements, forward and backward.
-  const uint32_t up1 = src_up<W>(1), down1 = src_down<W>(1);
-  const uint32_t pk_hi = bperm(up1, (uint32_t)(key[1] >> 32));
+  const uint32_t pk_hi = lane_up<W>(1, (uint32_t)(key[1] >> 32));
   bool head[2], valid[2];
   uint64_t bit[2];
 #pragma unroll
@@ -1331,34 +1450,32 @@ __device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t 
   uint64_t fv = f1;
 #pragma unroll
   for (uint32_t d = 1; d < W; d <<= 1) {
-    const uint32_t src = src_up<W>(d);
-    const uint64_t pv = bperm64(src, fv);
-    const uint32_t pf = bperm(src, ff ? 1u : 0u);
+    const uint64_t pv = lane_up64<W>(d, fv);
+    const uint32_t pf = lane_up<W>(d, ff ? 1u : 0u);
     const bool in = lane >= d;
     fv = (in && !ff) ? (fv | pv) : fv;
     ff = ff || (in && pf);
   }
   // every lane takes part in a permute (a disabled source lane reads as 0)
-  const uint64_t fv_up = bperm64(up1, fv);
+  const uint64_t fv_up = lane_up64<W>(1, fv);
   const uint64_t before = lane == 0 ? 0ull : fv_up;
   const uint64_t inc0 = head[0] ? bit[0] : (before | bit[0]);
   const uint64_t inc1 = head[1] ? bit[1] : (inc0 | bit[1]);
   // backward inclusive (a segment ends where the next element is a head)
-  const uint32_t nh0 = bperm(down1, head[0] ? 1u : 0u);
+  const uint32_t nh0 = lane_down<W>(1, head[0] ? 1u : 0u);
   const bool tail1 = lane == W - 1 || nh0;
   const bool tail0 = head[1];
   uint64_t bv = tail0 ? bit[0] : (bit[0] | bit[1]);
   bool bf = tail0 || tail1;
 #pragma unroll
   for (uint32_t d = 1; d < W; d <<= 1) {
-    const uint32_t src = src_down<W>(d);
-    const uint64_t pv = bperm64(src, bv);
-    const uint32_t pf = bperm(src, bf ? 1u : 0u);
+    const uint64_t pv = lane_down64<W>(d, bv);
+    const uint32_t pf = lane_down<W>(d, bf ? 1u : 0u);
     const bool in = lane + d < W;
     bv = (in && !bf) ? (bv | pv) : bv;
     bf = bf || (in && pf);
   }
-  const uint64_t bv_down = bperm64(down1, bv);
+  const uint64_t bv_down = lane_down64<W>(1, bv);
   const uint64_t after = lane == W - 1 ? 0ull : bv_down;
   const uint64_t suf1 = tail1 ? bit[1] : (after | bit[1]);
   const uint64_t suf0 = tail0 ? bit[0] : (suf1 | bit[0]);
@@ -1395,19 +1512,28 @@ __device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t 
   return kSmallSorted;
 }
 
+// Where a query's read lies in the read buffer, when the caller has loaded
+// it already (known), else loaded by process_query.
+struct QueryLoc {
+  uint64_t r0 = 0;
+  uint64_t len = 0;
+  bool known = false;
+};
+
 // Processes query q.  Returns kOverflow (uniformly) when the working set
 // exceeds the workspace capacities; nothing is written in that case.
 template <class TM>
 __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, const Caps &cap,
                                                   uint32_t q, bool last_resort,
-                                                  const uint8_t *comp) {
+                                                  const uint8_t *comp,
+                                                  const QueryLoc &loc = QueryLoc()) {
   constexpr uint32_t BS = TM::kSize;
   const uint32_t tid = TM::rank();
   const uint32_t read = q >> 1, strand = q & 1;
   const IndexView &ix = c.ix;
   const int T = ix.T, s = ix.s, k = ix.k;
-  const uint64_t r0 = c.rd.off[read];
-  const uint64_t m64 = c.rd.off[read + 1] - r0;
+  const uint64_t r0 = loc.known ? loc.r0 : c.rd.off[read];
+  const uint64_t m64 = loc.known ? loc.len : c.rd.off[read + 1] - r0;
   QueryOut res;
   res.highest = -1.0f;
   res.n_top = 0;
@@ -1432,34 +1558,39 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
   // Stage the strand: uppercase, reverse complement for strand 1
   // (Sequences.cpp:43-52, :795-841).  With probe records the base check is
   // done and staging waits until a seed needs the bases.
-  // The read is loaded as 4-byte aligned dwords (bytes one at a time at the
-  // buffer's ends); comp is the complement table in LDS, zero outside the
-  // nucleotide alphabet.
+  // The read is loaded as the 4-byte aligned dwords that hold its bytes,
+  // kStageWords per thread issued together, so that a read of up to
+  // 16 * BS bytes takes one memory round trip.  (Each dword holds a byte of
+  // the read, so it lies in the same memory page as that byte: the bytes
+  // beside the read that it also holds are never used.)  comp is the
+  // complement table in LDS, zero outside the nucleotide alphabet.
+  constexpr uint32_t kStageWords = 4;
   auto stage = [&]() {
     const uint8_t *first = c.rd.bytes + r0;
     const uintptr_t a0 = (uintptr_t)first & ~(uintptr_t)3;
     const uint32_t off = (uint32_t)((uintptr_t)first - a0);
     const uint32_t nw = (off + m + 3) >> 2;
-    const uintptr_t lo = (uintptr_t)c.rd.bytes, hi = lo + c.rd.off[c.rd.n];
     int b = 0;
-    for (uint32_t w = tid; w < nw; w += BS) {
-      const uintptr_t at = a0 + 4u * w;
-      uint32_t word = 0;
-      if (at >= lo && at + 4 <= hi) {
-        word = *(const uint32_t *)at;
-      } else {
-        for (uint32_t t = 0; t < 4; t++)
-          if (at + t >= lo && at + t < hi) word |= (uint32_t)(*(const uint8_t *)(at + t)) << (8 * t);
+    for (uint32_t w0 = tid; w0 < nw; w0 += kStageWords * BS) {
+      uint32_t word[kStageWords];
+#pragma unroll
+      for (uint32_t u = 0; u < kStageWords; u++) {
+        const uint32_t w = w0 + u * BS;
+        word[u] = w < nw ? *(const uint32_t *)(a0 + 4u * w) : 0u;
       }
 #pragma unroll
-      for (uint32_t t = 0; t < 4; t++) {
-        const uint32_t x = 4 * w + t - off;  // position in the read (wraps below 0)
-        if (x < m) {
-          const uint8_t ch = upper((uint8_t)(word >> (8 * t)));
-          const uint8_t cc = comp[ch];
-          b |= (cc == 0);
-          if (strand) ws.q[m - 1 - x] = cc;
-          else ws.q[x] = ch;
+      for (uint32_t u = 0; u < kStageWords; u++) {
+        const uint32_t w = w0 + u * BS;
+#pragma unroll
+        for (uint32_t t = 0; t < 4; t++) {
+          const uint32_t x = 4 * w + t - off;  // position in the read (wraps below 0)
+          if (w < nw && x < m) {
+            const uint8_t ch = upper((uint8_t)(word[u] >> (8 * t)));
+            const uint8_t cc = comp[ch];
+            b |= (cc == 0);
+            if (strand) ws.q[m - 1 - x] = cc;
+            else ws.q[x] = ch;
+          }
         }
       }
     }
@@ -2047,11 +2178,13 @@ constexpr uint32_t kQueueChunk = 16;
 // while the queue holds at least four chunks per team, one per team for
 // short queues (the overflow tiers).  Queries that do not fit are passed on
 // to the next tier.
-// Lane-team kernels are held to 128 VGPRs (4 wavefronts per SIMD): their
-// throughput follows the number of strands in flight.
+// Every tile kernel is held to 128 VGPRs (4 wavefronts per SIMD): their
+// throughput follows the number of strands in flight (the 64-thread teams
+// of 150 bp reads on an 8 Gbp index: 81 ms at 128 VGPRs against 91 ms at
+// 136, which leaves 3 wavefronts per SIMD).
 template <class TM>
 __global__ __launch_bounds__(TM::kSize * TM::kPerWave)
-__attribute__((amdgpu_waves_per_eu(TM::kPerWave > 1 ? 4 : 1, 8))) void k_classify(
+__attribute__((amdgpu_waves_per_eu(4, 8))) void k_classify(
     Ctx c, Caps cap, const uint32_t *qlist, const unsigned long long *qstart, uint32_t nq,
     const unsigned long long *nq_dev, unsigned long long *head) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -2060,6 +2193,10 @@ __attribute__((amdgpu_waves_per_eu(TM::kPerWave > 1 ? 4 : 1, 8))) void k_classif
   const Ws ws = ws_carve(smem + team * ws_stride_lds(cap), cap);
   __shared__ uint32_t s_base;
   __shared__ uint8_t s_comp[256];
+  // lane teams: a chunk's queries, read offsets and lengths
+  constexpr uint32_t kLoc = G > 1 ? kQueueChunk * G : 1;
+  __shared__ uint32_t s_q[kLoc], s_len[kLoc];
+  __shared__ uint64_t s_r0[kLoc];
   for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) s_comp[i] = kComp[i];
   __syncthreads();
   if (qlist && qstart) qlist += *qstart;
@@ -2091,13 +2228,43 @@ __attribute__((amdgpu_waves_per_eu(TM::kPerWave > 1 ? 4 : 1, 8))) void k_classif
     const uint32_t base = s_base;
     if (base == 0xFFFFFFFFu) break;
     const uint32_t end = (uint32_t)min((unsigned long long)base + chunk, n);
+    if constexpr (G > 1) {
+      // Lane teams (one wavefront per workgroup): the chunk's queries and
+      // where their reads lie, loaded by the whole wavefront at once, so
+      // that a team's queries do not each wait for these loads.
+      for (uint32_t k = threadIdx.x; k < chunk; k += blockDim.x) {
+        const uint32_t i = base + k;
+        uint32_t q = 0;
+        uint64_t r0 = 0, len = 0;
+        if (i < end) {
+          q = qlist ? qlist[i] : i;
+          if (!(q & kBigFlag)) {
+            r0 = c.rd.off[q >> 1];
+            len = c.rd.off[(q >> 1) + 1] - r0;
+          }
+        }
+        s_q[k] = q;
+        s_r0[k] = r0;
+        s_len[k] = (uint32_t)min(len, (uint64_t)0xFFFFFFFFu);
+      }
+      __syncthreads();
+    }
     for (uint32_t i = base + team; i < end; i += G) {
-      const uint32_t q = qlist ? qlist[i] : i;
+      QueryLoc loc;
+      uint32_t q;
+      if constexpr (G > 1) {
+        q = s_q[i - base];
+        loc.r0 = s_r0[i - base];
+        loc.len = s_len[i - base];
+        loc.known = true;
+      } else {
+        q = qlist ? qlist[i] : i;
+      }
       if (q & kBigFlag) {  // more entries than an LDS tier holds: straight on
         pass_on<TM>(c, q);
         continue;
       }
-      const uint32_t r = process_query<TM>(c, ws, cap, q, false, s_comp);
+      const uint32_t r = process_query<TM>(c, ws, cap, q, false, s_comp, loc);
       count_done<TM>(ws, r);
       after_query<TM>(c, q, r);
     }
@@ -2631,9 +2798,11 @@ constexpr uint32_t kBigEcap = 1u << 19;
 // its teams.
 Caps tile_caps(const mh_tile &t, int s, size_t lds_per_block) {
   // 1 KB stays free for the kernel's static LDS and the 16-byte rounding of
-  // each team's workspace
-  const size_t usable = lds_per_block > 2048 ? lds_per_block - 1024 : lds_per_block / 2;
-  return caps_for(t, s, usable / teams_of(t.block));
+  // each team's workspace, and lane teams' query chunk (16 bytes a query)
+  const uint32_t teams = teams_of(t.block);
+  const size_t reserve = 1024 + (teams > 1 ? 16ull * kQueueChunk * teams : 0);
+  const size_t usable = lds_per_block > 2 * reserve ? lds_per_block - reserve : lds_per_block / 2;
+  return caps_for(t, s, usable / teams);
 }
 
 // LDS of one workgroup of a tile.
