@@ -375,8 +375,12 @@ mh_status mh_get_tiles(const mh_handle *h, const mh_index *idx, mh_tile *tiles, 
  * NULL restores the defaults (sized from the first length tile).  Results
  * do not depend on them. */
 mh_status mh_set_overflow_tiles(mh_handle *h, const mh_tile *tiers);
-/* Times candidate tiles per length bucket on the given device-resident reads
- * and installs the fastest (the project's autotuner). */
+/* Times candidate tiles on the given device-resident reads (at most the
+ * first 2^20) and installs the fastest (the project's autotuner): per length
+ * bucket that holds strands, a local search from the current tile over half
+ * and double its team size and entry capacity, timed by the bucket's kernel
+ * plus the overflow tiers; then the two LDS overflow tiers the same way.
+ * iters: calls per candidate (0: 2), the fastest counts. */
 mh_status mh_autotune(mh_handle *h, const mh_index *idx, const uint8_t *d_reads,
                       const uint64_t *d_read_off, uint32_t n_reads, uint32_t iters);
 

@@ -7,11 +7,13 @@
 // the capacity, and both together) are timed by what the bucket costs: its
 // own kernel plus the overflow tiers, which take the strands a smaller tile
 // cannot hold (GPU events of one call, best of `iters` calls).  The search
-// moves to the best neighbour while one is faster by more than 1 %, at most
-// kMoves times per bucket.  The two LDS overflow tiers are searched the same
-// way last, by the tiers' time.  Team sizes 16 and 32 put several queries in
-// one wavefront and are tried for reads up to 1024 bases and capacities up
-// to 128 entries.  Results do not depend on the tiles: a query whose working
+// moves to the best neighbour while one is faster by more than 1 % and
+// 0.05 ms (timing noise), at most kMoves times per bucket.  The two LDS
+// overflow tiers are searched the same way last, by the tiers' time.  Calls
+// classify at most the first kTuneReads reads (a 200 ms call on 10M reads
+// becomes a 20 ms one).  Team sizes 16 and 32 put several queries in one
+// wavefront and are tried for reads up to 1024 bases and capacities up to
+// 128 entries.  Results do not depend on the tiles: a query whose working
 // set does not fit a tile's LDS goes to the overflow tiers, so only speed
 // changes.
 #include <hip/hip_runtime.h>
@@ -27,6 +29,7 @@ using namespace mh;
 namespace {
 
 constexpr int kMoves = 4;
+constexpr uint32_t kTuneReads = 1u << 20;
 
 bool allowed(const mh_tile &t, bool tier) {
   if (t.block < (tier ? 64u : 16u) || t.block > 1024 || t.entries_cap < 32 ||
@@ -54,6 +57,7 @@ extern "C" mh_status mh_autotune(mh_handle *h, const mh_index *idx, const uint8_
                                  const uint64_t *d_read_off, uint32_t n_reads, uint32_t iters) {
   if (!h || !idx || !n_reads || !d_reads || !d_read_off) return MH_ERR_ARG;
   if (!iters) iters = 2;
+  n_reads = std::min(n_reads, kTuneReads);
   hipSetDevice(h->device);
   mh_read_result *d_out = nullptr;
   uint32_t *d_top = nullptr;
@@ -133,7 +137,7 @@ extern "C" mh_status mh_autotune(mh_handle *h, const mh_index *idx, const uint8_
           pick = cand;
         }
       }
-      if (st || !(pick_ms < 0.99f * best_ms)) break;
+      if (st || !(pick_ms < std::min(0.99f * best_ms, best_ms - 0.05f))) break;
       tiles[b] = pick;
       best_ms = pick_ms;
     }
@@ -155,7 +159,7 @@ extern "C" mh_status mh_autotune(mh_handle *h, const mh_index *idx, const uint8_
           pick = cand;
         }
       }
-      if (st || !(pick_ms < 0.99f * best_ms)) break;
+      if (st || !(pick_ms < std::min(0.99f * best_ms, best_ms - 0.05f))) break;
       tiers[t] = pick;
       best_ms = pick_ms;
     }

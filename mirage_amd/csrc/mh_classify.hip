@@ -1312,13 +1312,40 @@ __device__ __forceinline__ uint64_t team_or64(uint64_t x) {
          seg_reduce<W, 2>((uint32_t)x);
 }
 
+// Bitonic network of N2 elements held two to a lane (element i: slot i & 1
+// of lane i >> 1), fully unrolled so that every partner distance is a
+// constant (cross-lane moves through DPP up to 8 lanes apart).
+template <uint32_t N2>
+__device__ __forceinline__ void lane_bitonic(uint64_t (&P)[2], uint32_t lane) {
+#pragma unroll
+  for (uint32_t k = 2; k <= N2; k <<= 1) {
+    const bool asc = ((2 * lane) & k) == 0;
+#pragma unroll
+    for (uint32_t d = k >> 1; d > 1; d >>= 1) {
+      const uint32_t lx = d >> 1;  // partner lane = lane ^ lx
+      // the lower lane of a pair keeps the minimum in an ascending run
+      const bool want_min = ((lane & lx) == 0) == asc;
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const uint64_t pk = xor_lane64_any(lx, P[j]);
+        P[j] = ((pk < P[j]) == want_min) ? pk : P[j];
+      }
+    }
+    // d == 1: the two slots of a lane
+    const uint64_t lo = P[0] < P[1] ? P[0] : P[1], hi = P[0] < P[1] ? P[1] : P[0];
+    P[0] = asc ? lo : hi;
+    P[1] = asc ? hi : lo;
+  }
+}
+
 // Small working sets (at most 64 entries and 64 hits, fewer than 2^26
 // references, a team inside one wavefront) in registers: element i of the
 // team's array is slot i & 1 of lane i >> 1.  Gives the decisions of the LDS
 // path below.
-//  * Direct hit: tested on the gathered entries with ballots (r* from the
-//    first bucket: as many entries as hits, all on its diagonal; no other
-//    reference of the second bucket present in every bucket 1..N-2).
+//  * Direct hit (whole-wavefront teams): tested on the gathered entries with
+//    ballots (r* from the first bucket: as many entries as hits, all on its
+//    diagonal; no other reference of the second bucket present in every
+//    bucket 1..N-2).
 //  * Otherwise the entries are sorted by (ref, refPos, qPos) with a bitonic
 //    network across the lanes; a segmented OR-scan over each reference's
 //    run of elements gives its hit mask (bit i: an entry at hit index i),
@@ -1349,7 +1376,10 @@ __device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t 
       qp[j] = ws.hit_j[hit[j]];
     }
   }
-  if (!(c.flags & MH_CLASSIFY_NO_DIRECT) && nhits == nprobe && nhits >= 3) {
+  // (Lane teams skip the direct test: with their few entries the register
+  // sort below costs less than the test; 2.49 against 2.55 ms for 1M 150 bp
+  // reads on a 5000-reference index.)
+  if (W == 64 && !(c.flags & MH_CLASSIFY_NO_DIRECT) && nhits == nprobe && nhits >= 3) {
     const uint32_t N = nhits;
     const uint32_t c0 = ws.hit_base[1];  // entries of the first bucket
     uint32_t nstar = 0, rstar = 0;
@@ -1410,23 +1440,13 @@ __device__ uint32_t small_path(const Ctx &c, const Ws &ws, uint32_t E, uint32_t 
 #pragma unroll
   for (int j = 0; j < 2; j++)
     P[j] = ok[j] ? ((uint64_t)ref[j] << 38) | ((uint64_t)pos[j] << 6) | hit[j] : ~0ull;
-  const uint32_t n2 = next_pow2(E < 2 ? 2 : E);
-  for (uint32_t k = 2; k <= n2; k <<= 1) {
-    const bool asc = ((2 * lane) & k) == 0;
-    for (uint32_t d = k >> 1; d > 1; d >>= 1) {
-      const uint32_t lx = d >> 1;  // partner lane = lane ^ lx
-      // the lower lane of a pair keeps the minimum in an ascending run
-      const bool want_min = ((lane & lx) == 0) == asc;
-#pragma unroll
-      for (int j = 0; j < 2; j++) {
-        const uint64_t pk = xor_lane64_any(lx, P[j]);
-        P[j] = ((pk < P[j]) == want_min) ? pk : P[j];
-      }
-    }
-    // d == 1: the two slots of a lane
-    const uint64_t lo = P[0] < P[1] ? P[0] : P[1], hi = P[0] < P[1] ? P[1] : P[0];
-    P[0] = asc ? lo : hi;
-    P[1] = asc ? hi : lo;
+  switch (next_pow2(E < 2 ? 2 : E)) {
+    case 2: lane_bitonic<2>(P, lane); break;
+    case 4: lane_bitonic<4>(P, lane); break;
+    case 8: lane_bitonic<8>(P, lane); break;
+    case 16: lane_bitonic<16>(P, lane); break;
+    case 32: lane_bitonic<32>(P, lane); break;
+    default: lane_bitonic<64>(P, lane); break;
   }
   uint64_t key[2];  // ref << 32 | refPos
 #pragma unroll
@@ -1571,15 +1591,18 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
     const uint32_t off = (uint32_t)((uintptr_t)first - a0);
     const uint32_t nw = (off + m + 3) >> 2;
     int b = 0;
-    for (uint32_t w0 = tid; w0 < nw; w0 += kStageWords * BS) {
+    for (uint32_t wb = 0; wb < nw; wb += kStageWords * BS) {  // team-uniform
+      const uint32_t w0 = wb + tid;
       uint32_t word[kStageWords];
 #pragma unroll
       for (uint32_t u = 0; u < kStageWords; u++) {
+        if (wb + u * BS >= nw) break;
         const uint32_t w = w0 + u * BS;
         word[u] = w < nw ? *(const uint32_t *)(a0 + 4u * w) : 0u;
       }
 #pragma unroll
       for (uint32_t u = 0; u < kStageWords; u++) {
+        if (wb + u * BS >= nw) break;
         const uint32_t w = w0 + u * BS;
 #pragma unroll
         for (uint32_t t = 0; t < 4; t++) {
