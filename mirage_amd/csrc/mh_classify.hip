@@ -702,19 +702,54 @@ __device__ __forceinline__ float matrix_score(uint64_t sum_r, uint64_t sum_c, ui
   return (float)(((double)sf / 100.0) * (double)pct);
 }
 
+// BL (4 or 8) consecutive bytes from p (any alignment) as BL / 4
+// little-endian words, read as the aligned dwords that hold them (one more
+// than BL / 4); dwords below lo read as 0.
+template <uint32_t BL>
+__device__ __forceinline__ void load_bytes(const uint8_t *p, uintptr_t lo, uint32_t (&w)[BL / 4]) {
+  constexpr uint32_t ND = BL / 4 + 1;
+  const uintptr_t a = (uintptr_t)p & ~(uintptr_t)3;
+  const uint32_t sh = (uint32_t)((uintptr_t)p & 3);
+  uint32_t d[ND];
+#pragma unroll
+  for (uint32_t i = 0; i < ND; i++) d[i] = a + 4 * i >= lo ? *(const uint32_t *)(a + 4 * i) : 0u;
+#pragma unroll
+  for (uint32_t i = 0; i < BL / 4; i++) w[i] = __builtin_amdgcn_alignbyte(d[i + 1], d[i], sh);
+}
+
+// Bit 8j + 7 set where byte j of the BL bytes in a and b differs.
+template <uint32_t BL>
+__device__ __forceinline__ uint64_t byte_diff(const uint32_t (&a)[BL / 4], const uint32_t (&b)[BL / 4]) {
+  uint64_t nz = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < BL / 4; i++) {
+    const uint32_t x = a[i] ^ b[i];
+    nz |= (uint64_t)((((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u) << (32 * i);
+  }
+  return nz;
+}
+
 // Head/tail extension of one seed by one seed worker (MatrixCPU.cpp:93-172).
 // Returns the matched length tot and the head length h; all lanes agree.
 // Tail and head are independent, so each round loads a chunk of both before
-// the ballots find the first mismatch: every lane loads 64 / kSeedLanes
+// the ballots find the first mismatch: every lane takes 64 / kSeedLanes
 // consecutive bases in each direction, so one memory round trip covers 64
-// bases (the whole extension at T <= 64).  The first mismatch is the lowest
-// position of the first lane whose bases hold one; that lane's position is
-// read with a cross-lane permute.
+// bases (the whole extension at T <= 64).  A lane reads its reference and
+// query bases as words (the aligned dwords that hold them; a reference's
+// bytes are followed by 64 bytes of padding, and nothing below the bytes
+// buffer `lo` is read) and finds its first differing base with byte
+// arithmetic (query words below the query's first byte, which is 4-byte
+// aligned and followed by more of its workspace, read as 0).  The head's
+// bases l = l0 .. l0 + BL - 1 lie at descending
+// positions, so its word block holds base l at byte BL - 1 - (l - l0).  The
+// first mismatch is the lowest position of the first lane whose bases hold
+// one; that lane's position is read with a cross-lane permute.
 template <class TM>
 __device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const uint8_t *rb,
                                             uint64_t ref_len, uint32_t sref, uint32_t rs,
                                             uint32_t lref, uint32_t re, int T, bool ultrafast,
-                                            uint32_t &tot, uint32_t &head, uint32_t &compares) {
+                                            const uint8_t *lo, uint32_t &tot, uint32_t &head,
+                                            uint32_t &compares) {
   constexpr uint32_t W = TM::kSeedLanes;
   const uint32_t lane = seed_lane<TM>();
   const uint32_t mid = re - rs;
@@ -741,28 +776,32 @@ __device__ __forceinline__ void extend_seed(const uint8_t *q, uint32_t m, const 
     h = lb;
     bool tdone = tl == 0, hdone = hl == 0;
     constexpr uint32_t BL = 64 / W;  // bases per lane, direction and round
+    static_assert(BL == 4 || BL == 8, "4 or 8 bases per lane");
     constexpr uint32_t CH = BL * W;
+    constexpr uint64_t kAll = BL == 8 ? ~0ull : 0xFFFFFFFFull;
     for (uint32_t b = 0; !(tdone && hdone); b += CH) {
       const uint32_t l0 = b + BL * lane;
-      uint8_t rt[BL], rh[BL];
-#pragma unroll
-      for (uint32_t j = 0; j < BL; j++) {  // every load issues before any compare
-        const uint32_t l = l0 + j;
-        rt[j] = (!tdone && l < tl) ? rb[(uint64_t)lref + l] : 0;
-        rh[j] = (!hdone && l < hl) ? rb[sref - l] : 0;
+      uint64_t tm = 0, hm = 0;  // bit 8j + 7: base j of the lane's block differs
+      if (!tdone && l0 < tl) {
+        uint32_t rw[BL / 4], qw[BL / 4];
+        load_bytes<BL>(rb + (uint64_t)lref + l0, (uintptr_t)lo, rw);
+        load_bytes<BL>(q + re + l0, (uintptr_t)q, qw);
+        const uint32_t nv = tl - l0 < BL ? tl - l0 : BL;
+        tm = byte_diff<BL>(rw, qw) & (nv >= BL ? kAll : ((1ull << (8 * nv)) - 1));
       }
-      uint32_t tm = 0, hm = 0;  // bit j: base l0 + j differs
-#pragma unroll
-      for (uint32_t j = 0; j < BL; j++) {
-        const uint32_t l = l0 + j;
-        if (!tdone && l < tl && q[re + l] != rt[j]) tm |= 1u << j;
-        if (!hdone && l < hl && q[rs - 1 - l] != rh[j]) hm |= 1u << j;
+      if (!hdone && l0 < hl) {
+        // reference bytes sref-l0-BL+1 .. sref-l0, query rs-l0-BL .. rs-1-l0
+        uint32_t rw[BL / 4], qw[BL / 4];
+        load_bytes<BL>(rb + (int64_t)sref - (int64_t)l0 - (int64_t)(BL - 1), (uintptr_t)lo, rw);
+        load_bytes<BL>(q + (int64_t)rs - (int64_t)l0 - (int64_t)BL, (uintptr_t)q, qw);
+        const uint32_t nv = hl - l0 < BL ? hl - l0 : BL;  // bytes BL-nv .. BL-1
+        hm = byte_diff<BL>(rw, qw) & (kAll & ~((1ull << (8 * (BL - nv))) - 1));
       }
       const unsigned long long bt = seed_ballot<TM>(tm != 0);
       const unsigned long long bh = seed_ballot<TM>(hm != 0);
-      // each lane's first differing position (read from the first lane that has one)
-      const uint32_t pt = l0 + (tm ? (uint32_t)__ffs(tm) - 1 : 0u);
-      const uint32_t ph = l0 + (hm ? (uint32_t)__ffs(hm) - 1 : 0u);
+      // each lane's first differing base (read from the first lane that has one)
+      const uint32_t pt = l0 + (tm ? (uint32_t)(__ffsll((long long)tm) - 1) >> 3 : 0u);
+      const uint32_t ph = l0 + (hm ? BL - 1 - ((63u - (uint32_t)__clzll((long long)hm)) >> 3) : 0u);
       if (!tdone) {
         if (bt) {
           t = bperm(src_at<W>((uint32_t)__ffsll((long long)bt) - 1), pt);
@@ -806,7 +845,7 @@ __device__ __forceinline__ void set_bits(uint32_t *bm, uint32_t a, uint32_t len)
 // Returns on all threads.
 template <class TM, class GetSeed>
 __device__ void score_ref(const uint8_t *q, uint32_t m, const uint8_t *rb, uint64_t ref_len,
-                          uint32_t n_seeds, int T, bool ultrafast, const Ws &ws,
+                          const uint8_t *lo, uint32_t n_seeds, int T, bool ultrafast, const Ws &ws,
                           GetSeed get_seed, uint64_t &sum_r, uint64_t &sum_c, uint32_t &hits) {
   constexpr uint32_t NW = TM::kSize / TM::kSeedLanes, BS = TM::kSize;
   const uint32_t wid = TM::rank() / TM::kSeedLanes, lane = seed_lane<TM>();
@@ -825,7 +864,7 @@ __device__ void score_ref(const uint8_t *q, uint32_t m, const uint8_t *rb, uint6
     const uint32_t rs = sq, re = lq;
     if (rs > re) continue;  // MatrixCPU.cpp:57-58
     uint32_t tot, h, cmp;
-    extend_seed<TM>(q, m, rb, ref_len, sref, rs, lref, re, T, ultrafast, tot, h, cmp);
+    extend_seed<TM>(q, m, rb, ref_len, sref, rs, lref, re, T, ultrafast, lo, tot, h, cmp);
     ncmp += (unsigned long long)wt * cmp;
     if (tot >= (uint32_t)T) {  // MatrixCPU.cpp:178-184
       sc += (unsigned long long)wt * tot;
@@ -957,8 +996,8 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
         const uint32_t rs = sq, re = lq;
         if (rs > re) continue;  // MatrixCPU.cpp:57-58
         uint32_t tl, h, cmp;
-        extend_seed<TM>(ws.q, m, ix.bytes + sl.ro, sl.len, sref, rs, lref, re, T, ultrafast, tl, h,
-                        cmp);
+        extend_seed<TM>(ws.q, m, ix.bytes + sl.ro, sl.len, sref, rs, lref, re, T, ultrafast,
+                        ix.bytes, tl, h, cmp);
         ncmp += (unsigned long long)wt * cmp;
         if (tl >= (uint32_t)T) {  // MatrixCPU.cpp:178-184
           if (lane == 0) {
@@ -2068,8 +2107,8 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
     };
     uint64_t sum_r, sum_c;
     uint32_t hits;
-    score_ref<TM>(ws.q, m, rb, ref_len, c1 - c0 + nvirt, T, ultrafast, ws, seed, sum_r, sum_c,
-                  hits);
+    score_ref<TM>(ws.q, m, rb, ref_len, ix.bytes, c1 - c0 + nvirt, T, ultrafast, ws, seed, sum_r,
+                  sum_c, hits);
     const float score = matrix_score(sum_r, sum_c, hits, m);
     float rel = -1.0f;
     uint32_t valid = 0, lut_bad = 0;
@@ -2615,8 +2654,9 @@ __global__ __launch_bounds__(64) void k_score_seeds(IndexView ix, Luts lut, cons
         wt = 1;
       };
       if (s1 > s0) {
-        score_ref<BlockTeam<64>>(ws.q, m, ix.bytes + ro, ref_len, (uint32_t)(s1 - s0), ix.T,
-                      (flags & MH_CLASSIFY_ULTRAFAST) != 0, ws, seed, sum_r, sum_c, hits);
+        score_ref<BlockTeam<64>>(ws.q, m, ix.bytes + ro, ref_len, ix.bytes, (uint32_t)(s1 - s0),
+                                 ix.T, (flags & MH_CLASSIFY_ULTRAFAST) != 0, ws, seed, sum_r,
+                                 sum_c, hits);
         score = matrix_score(sum_r, sum_c, hits, m);
         if (score > -1.0f && !rel_score(lut, m, ref_len, score, rel)) rel = -1.0f;
       }
