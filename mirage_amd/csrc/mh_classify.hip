@@ -693,6 +693,18 @@ __device__ __forceinline__ bool rel_score(const Luts &lut, uint32_t m, uint64_t 
   return true;
 }
 
+// The same with L = min(m, reference length) given.
+__device__ __forceinline__ bool rel_score_l(const Luts &lut, uint32_t L, float score, float &rel) {
+  const uint32_t xi = __float_as_uint(score + 1.0f) - 0x3f800000u;
+  if (xi >= lut.n_x || L >= lut.n_smax) return false;
+  rel = lut.x_log[xi] / lut.smax_log[L];
+  return true;
+}
+
+// A scored reference whose relative score is still to be looked up is
+// marked kRelPending + L in valid_of.
+constexpr uint32_t kRelPending = 2;
+
 // Score = (sumR / sumC) / 100 * percentageMatch (MatrixCPU.cpp:195-213).
 __device__ __forceinline__ float matrix_score(uint64_t sum_r, uint64_t sum_c, uint32_t hits,
                                               uint32_t m) {
@@ -920,13 +932,21 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
     const bool f = r < nranks && ws.t3[r] == G;
     uint32_t tot;
     const uint32_t idx = TM::excl_scan(f ? 1u : 0u, ws.red, tot);
+    // This thread's reference, where its bytes lie: loaded once, before the
+    // rounds, so that the loads of every round's references are in flight
+    // together.
+    uint32_t R = 0;
+    uint64_t ro = 0, rlen = 0;
+    if (f) {
+      R = ws.rref[r];
+      ro = ix.ref_off[R];
+      rlen = ix.ref_off[R + 1] - ro;
+    }
     for (uint32_t g = 0; g < tot; g += cap.nslot) {
       const uint32_t nb = min(cap.nslot, tot - g);
       TM::sync();
       if (f && idx >= g && idx < g + nb) {
         RefSlot &sl = ws.slots[idx - g];
-        const uint32_t R = ws.rref[r];
-        const uint64_t ro = ix.ref_off[R];
         const uint32_t c0 = ws.v1[r];
         const uint32_t c1 = (r + 1 < nranks) ? ws.v1[r + 1] : nchains;
         // reference 0 also gets the unstored copies of (0, 0) as one-pair
@@ -935,7 +955,7 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
         sl.acc[0] = 0;
         sl.acc[1] = 0;
         sl.ro = ro;
-        sl.len = ix.ref_off[R + 1] - ro;
+        sl.len = rlen;
         sl.r = r;
         sl.R = R;
         sl.c0 = c0;
@@ -1010,13 +1030,20 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
       }
       if (lane == 0 && ncmp && ws.stats) atomicAdd(&ws.stats[kExt], ncmp);
       TM::sync();
-      // one thread per slot: row hits, Score, relative score (the table
-      // lookups of every slot in flight together)
+      // one thread per slot: row hits, Score; without the candidate output
+      // the relative score waits for the pass after the last round (the table
+      // lookups of every scored reference in flight together)
       for (uint32_t b = tid; b < nb; b += BS) {
         const uint32_t *bm = ws.slot_bm + b * bw;
         uint32_t hits = 0;
         for (uint32_t w = 0; w < nw; w++) hits += __popc(bm[w]);
-        {
+        if (!c.out.cands) {
+          const RefSlot &sl = ws.slots[b];
+          const float score = matrix_score(sl.acc[0], sl.acc[1], hits, m);
+          rel_of[sl.r] = score;
+          // MatrixCPU.cpp:356; pending with L = min(m, reference length)
+          valid_of[sl.r] = score > -1.0f ? kRelPending + (uint32_t)min((uint64_t)m, sl.len) : 0u;
+        } else {
           const RefSlot &sl = ws.slots[b];
           const float score = matrix_score(sl.acc[0], sl.acc[1], hits, m);
           float rel = -1.0f;
@@ -1027,25 +1054,40 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
           }
           rel_of[sl.r] = rel;
           valid_of[sl.r] = valid;
-          if (c.out.cands) {
-            const unsigned long long slot = atomicAdd(&c.out.ctr->cand_used, 1ull);
-            if (slot < c.out.cand_cap) {
-              mh_cand cd;
-              cd.read = read;
-              cd.strand = strand;
-              cd.ref = sl.R;
-              cd.hits = hits;
-              cd.sum_r = sl.acc[0];
-              cd.sum_c = sl.acc[1];
-              cd.score = score;
-              cd.rel = rel;
-              cd.n_seeds = (uint32_t)(sl.nchain +
-                                      (unsigned long long)(sl.nseed - sl.nchain) * ix.phantom);
-              cd.pad = 0;
-              c.out.cands[slot] = cd;
-            }
+          const unsigned long long slot = atomicAdd(&c.out.ctr->cand_used, 1ull);
+          if (slot < c.out.cand_cap) {
+            mh_cand cd;
+            cd.read = read;
+            cd.strand = strand;
+            cd.ref = sl.R;
+            cd.hits = hits;
+            cd.sum_r = sl.acc[0];
+            cd.sum_c = sl.acc[1];
+            cd.score = score;
+            cd.rel = rel;
+            cd.n_seeds = (uint32_t)(sl.nchain +
+                                    (unsigned long long)(sl.nseed - sl.nchain) * ix.phantom);
+            cd.pad = 0;
+            c.out.cands[slot] = cd;
           }
         }
+      }
+    }
+  }
+  if (!c.out.cands) {
+    // relative scores of the pending references (MatrixResult.cpp:21-29)
+    TM::sync();
+    for (uint32_t r = tid; r < nranks; r += BS) {
+      const uint32_t v = valid_of[r];
+      if (v >= kRelPending) {
+        float rel = -1.0f;
+        if (rel_score_l(c.lut, v - kRelPending, rel_of[r], rel)) {
+          valid_of[r] = 1;
+        } else {
+          valid_of[r] = 0;
+          ws.red[kBcast + 3] = 1;
+        }
+        rel_of[r] = rel;
       }
     }
   }
@@ -2290,10 +2332,23 @@ __attribute__((amdgpu_waves_per_eu(4, 8))) void k_classify(
     const uint32_t base = s_base;
     if (base == 0xFFFFFFFFu) break;
     const uint32_t end = (uint32_t)min((unsigned long long)base + chunk, n);
-    if constexpr (G > 1) {
-      // Lane teams (one wavefront per workgroup): the chunk's queries and
-      // where their reads lie, loaded by the whole wavefront at once, so
-      // that a team's queries do not each wait for these loads.
+    // The chunk's queries and where their reads lie are loaded at once, so
+    // that a query does not wait for these two dependent loads of its own:
+    // whole-workgroup teams hold them in registers, lane k of every
+    // wavefront the chunk's k-th query (chunk <= kQueueChunk <= 64); lane
+    // teams in LDS.
+    uint32_t c_q = 0, c_len = 0;
+    uint64_t c_r0 = 0;
+    if constexpr (G == 1) {
+      const uint32_t k = threadIdx.x & 63;
+      if (base + k < end) {
+        c_q = qlist ? qlist[base + k] : base + k;
+        if (!(c_q & kBigFlag)) {
+          c_r0 = c.rd.off[c_q >> 1];
+          c_len = (uint32_t)min(c.rd.off[(c_q >> 1) + 1] - c_r0, (uint64_t)0xFFFFFFFFu);
+        }
+      }
+    } else {
       for (uint32_t k = threadIdx.x; k < chunk; k += blockDim.x) {
         const uint32_t i = base + k;
         uint32_t q = 0;
@@ -2314,14 +2369,17 @@ __attribute__((amdgpu_waves_per_eu(4, 8))) void k_classify(
     for (uint32_t i = base + team; i < end; i += G) {
       QueryLoc loc;
       uint32_t q;
-      if constexpr (G > 1) {
+      if constexpr (G == 1) {
+        const uint32_t k = i - base;  // uniform
+        q = readlane(c_q, k);
+        loc.r0 = ((uint64_t)readlane((uint32_t)(c_r0 >> 32), k) << 32) | readlane((uint32_t)c_r0, k);
+        loc.len = readlane(c_len, k);
+      } else {
         q = s_q[i - base];
         loc.r0 = s_r0[i - base];
         loc.len = s_len[i - base];
-        loc.known = true;
-      } else {
-        q = qlist ? qlist[i] : i;
       }
+      loc.known = true;
       if (q & kBigFlag) {  // more entries than an LDS tier holds: straight on
         pass_on<TM>(c, q);
         continue;

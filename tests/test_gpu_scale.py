@@ -52,11 +52,13 @@ def test_repetitive_index_matches_oracle(handle, repetitive, mode):
             handle.set_overflow_tiles([(256, 64, 32, 0), (handle.max_read_len, 64, 32, 0)])
         out, top, cands = ix.classify_host(qb, qo, flags=flags, want_cands=True)
         rows = handle.kernel_counters()
+        nout, ntop, _ = ix.classify_host(qb, qo, flags=flags)
     finally:
         handle.set_tiles(None)
         handle.set_overflow_tiles(None)
         ix.free()
     compare_reads(out, top, oout, otop, f"repetitive {mode}")
+    compare_reads(nout, ntop, oout, otop, f"repetitive {mode}, no candidate output")
     compare_cands(cands, ocands, f"repetitive {mode}")
     assert max(len(t) for t in native.tie_sets(out, top)) > 16  # segment ties
     if mode == "hbm_tiers":

@@ -225,12 +225,16 @@ def test_workgroup_teams_score_many_references(handle, flags, table):
                           for ml, b, e, g in WORKGROUP_TABLES[table]])
         ix = handle.build_index(rb, ro, 30)
         out, top, cands = ix.classify_host(qb, qo, flags=flags, want_cands=True, top_cap=64 * 240)
+        # without the candidate output the relative scores are looked up
+        # after the last round of slots
+        nout, ntop, _ = ix.classify_host(qb, qo, flags=flags, top_cap=64 * 240)
         ix.free()
     finally:
         handle.set_tiles(None)
     oix = O.Index(rb, ro, 30)
     oout, otop, ocands = oix.classify(qb, qo, want_cands=True, top_cap=64)
     compare_reads(out, top, oout, otop, f"workgroup teams {table}")
+    compare_reads(nout, ntop, oout, otop, f"workgroup teams {table}, no candidate output")
     compare_cands(cands, ocands, f"workgroup teams {table}")
     per_strand = np.unique(cands[["read", "strand"]], return_counts=True)[1]
     assert per_strand.max() > 32  # three or more batches of slots
