@@ -220,8 +220,13 @@ mh_status mh_index_save_mhm(mh_handle *h, const mh_index *idx, const char *path,
 
 /* ---- classification ---------------------------------------------------- */
 
+/* Most reads one classify call takes, plus one: a strand's id (2 * read +
+ * strand) and a flag share 32 bits.  Larger calls return MH_ERR_ARG. */
+#define MH_MAX_CALL_READS 1073741824u /* 2^30 */
+
 /* Full path on device-resident reads: d_reads bytes, d_read_off n_reads+1
- * offsets.  Results go to d_out[n_reads] and the top-tie pool d_top_pool
+ * offsets (n_reads below MH_MAX_CALL_READS).  Results go to d_out[n_reads]
+ * and the top-tie pool d_top_pool
  * (capacity top_cap ids).  d_cands (may be NULL) receives every seeded
  * candidate of both strands, in no particular order.  On MH_ERR_OVERFLOW,
  * *top_needed / *cand_needed hold the sizes required.  Asynchronous on the

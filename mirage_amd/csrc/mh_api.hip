@@ -281,6 +281,11 @@ static mh_status classify_on(mh_handle *h, Exec &x, const mh_index *idx, const u
                              mh_cand *d_cands, uint64_t cand_cap, bool wait,
                              uint64_t *top_needed, uint64_t *cand_needed,
                              const FixedScratch *fixed = nullptr) {
+  // A strand's id (2 * read + strand) and its overflow flag share 32 bits.
+  if (n_reads >= MH_MAX_CALL_READS)
+    return set_error(h, MH_ERR_ARG,
+                     "classify: " + std::to_string(n_reads) + " reads, at most " +
+                         std::to_string(MH_MAX_CALL_READS - 1) + " per call");
   ClassifyArgs a;
   a.ix = idx->view();
   a.luts = h->luts();

@@ -2275,6 +2275,8 @@ __device__ __forceinline__ void after_query(const Ctx &c, uint32_t q, uint32_t r
 
 // Most queries a team takes from its queue at a time.
 constexpr uint32_t kQueueChunk = 16;
+// No query to pass on (query ids are below 2^31 - 2: fewer than 2^30 reads).
+constexpr uint32_t kNoPass = 0xFFFFFFFFu;
 
 // Length-tile kernel.  Each team has its own workspace in LDS.  Plain form:
 // one query per team; batched form: a persistent loop over a queue of
@@ -2337,7 +2339,7 @@ __attribute__((amdgpu_waves_per_eu(4, 8))) void k_classify(
     // whole-workgroup teams hold them in registers, lane k of every
     // wavefront the chunk's k-th query (chunk <= kQueueChunk <= 64); lane
     // teams in LDS.
-    uint32_t c_q = 0, c_len = 0;
+    uint32_t c_q = kNoPass, c_len = 0;
     uint64_t c_r0 = 0;
     if constexpr (G == 1) {
       const uint32_t k = threadIdx.x & 63;
@@ -2381,12 +2383,38 @@ __attribute__((amdgpu_waves_per_eu(4, 8))) void k_classify(
       }
       loc.known = true;
       if (q & kBigFlag) {  // more entries than an LDS tier holds: straight on
-        pass_on<TM>(c, q);
+        if constexpr (G > 1) pass_on<TM>(c, q);  // workgroup teams: lane k keeps it
         continue;
       }
       const uint32_t r = process_query<TM>(c, ws, cap, q, false, s_comp, loc);
       count_done<TM>(ws, r);
-      after_query<TM>(c, q, r);
+      if constexpr (G == 1) {
+        // lane k of every wavefront now holds what query k passes on
+        if ((threadIdx.x & 63) == i - base)
+          c_q = r == kOverflow ? q : r == kOverflowBig ? q | kBigFlag : kNoPass;
+      } else {
+        after_query<TM>(c, q, r);
+      }
+    }
+    if constexpr (G == 1) {
+      // The chunk's queries for the next tier, appended with one atomic per
+      // chunk: one counter takes every overflow of the launch, and a
+      // returning atomic per query there made the queries wait in turn.
+      if (threadIdx.x < 64) {
+        const bool pass = c_q != kNoPass;
+        const uint64_t bal = __ballot(pass);
+        if (bal) {
+          unsigned long long at = 0;
+          if (threadIdx.x == 0)
+            at = atomicAdd(c.out.overflow_cnt, (unsigned long long)__popcll(bal));
+          at = ((unsigned long long)readlane((uint32_t)(at >> 32), 0) << 32) |
+               readlane((uint32_t)at, 0);
+          const uint32_t before =
+              __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+          if (pass && at + before < c.out.overflow_cap) c.out.overflow[at + before] = c_q;
+        }
+      }
     }
   }
   stats_flush<TM>(ws, c.out.ctr, slot, c.kid);

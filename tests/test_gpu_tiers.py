@@ -185,6 +185,20 @@ def test_invalid_tile_tables_are_refused(handle):
     ix.free()
 
 
+def test_calls_of_2_30_reads_are_refused(handle):
+    """Strand ids (2 * read + strand) and the overflow flag share 32 bits: a
+    call of 2^30 reads is refused before the device is touched."""
+    import torch
+    w = D.workload(1, 0.01)
+    rb, ro, _, _, _ = D.generate(w)
+    ix = handle.build_index(rb, ro, w.threshold)
+    one = torch.zeros(64, dtype=torch.uint8, device="cuda")
+    with pytest.raises(native.MirageError) as e:
+        ix.classify_device(one, one, 1 << 30, one, one, 1)
+    assert e.value.status == native.MH_ERR_ARG and "per call" in str(e.value)
+    ix.free()
+
+
 WORKGROUP_TABLES = {
     "wg256": [(200, 256, 1024, 0), (None, 256, 1024, 0)],
     "wg128": [(200, 128, 512, 0), (None, 128, 512, 0)],
