@@ -189,12 +189,13 @@ def serve_stream(native, ix, reqs, sizes, loads, rng, log, max_batch_reads=65536
                     time.sleep(0.0001)
                 need, inf = srv.wait_into(ids[i], out, top)  # raises on a failed request
                 assert need <= len(top)
-                lat.append((inf["done_s"] - inf["submit_s"], inf["submit_s"], inf["done_s"]))
+                lat.append((inf["done_s"] - inf["submit_s"], inf["submit_s"], inf["done_s"],
+                            inf["admit_s"] - inf["submit_s"], inf["done_s"] - inf["admit_s"]))
 
         th = threading.Thread(target=waiter)
         th.start()
         gaps = rng.exponential(1.0 / rate, len(reqs)) if rate else np.zeros(len(reqs))
-        nxt = time.perf_counter()
+        nxt = t_sub = time.perf_counter()
         for (b, o), g in zip(reqs, gaps):
             if rate:
                 nxt += g
@@ -202,17 +203,32 @@ def serve_stream(native, ix, reqs, sizes, loads, rng, log, max_batch_reads=65536
                 if d > 0:
                     time.sleep(d)
             ids.append(srv.submit_arrays(b, o))
+        t_sub = time.perf_counter() - t_sub
         th.join()
         srv.stop()
         lt = np.array([x[0] for x in lat])
+        wt = np.array([x[3] for x in lat])
+        st = np.array([x[4] for x in lat])
         span = max(x[2] for x in lat) - min(x[1] for x in lat)
-        return {"reads_per_s": float(sizes.sum() / span),
+        # where the latency goes: waiting for a slot and HBM budget (submit
+        # -> admit), then the batch's copies, kernels and hand-back (admit ->
+        # done); at the 99th percentile request, its own split
+        i99 = int(np.argsort(lt)[min(len(lt) - 1, int(np.ceil(0.99 * len(lt))) - 1)])
+        return {"reads_per_s": float(sizes.sum() / span), "span_ms": span * 1e3,
+                "submit_ms": t_sub * 1e3,
                 "p50_ms": float(np.percentile(lt, 50) * 1e3),
-                "p99_ms": float(np.percentile(lt, 99) * 1e3), "requests": len(reqs)}
+                "p99_ms": float(np.percentile(lt, 99) * 1e3), "requests": len(reqs),
+                "wait_p50_ms": float(np.percentile(wt, 50) * 1e3),
+                "wait_p99_ms": float(np.percentile(wt, 99) * 1e3),
+                "service_p50_ms": float(np.percentile(st, 50) * 1e3),
+                "service_p99_ms": float(np.percentile(st, 99) * 1e3),
+                "p99_request": {"reads": int(sizes[i99]), "wait_ms": float(wt[i99] * 1e3),
+                                "service_ms": float(st[i99] * 1e3)}}
 
     run(0)  # warm-up: slots, streams, first launches
     sat = run(0)
-    log(f"serving saturation: {sat['reads_per_s'] / 1e6:.2f} M reads/s, p50 {sat['p50_ms']:.1f} ms")
+    log(f"serving saturation: {sat['reads_per_s'] / 1e6:.2f} M reads/s, p50 {sat['p50_ms']:.1f} ms "
+        f"({sat['span_ms']:.1f} ms, of which submitting {sat['submit_ms']:.1f} ms)")
     out = {"saturation": sat, "loads": [], "max_batch_reads": max_batch_reads,
            "max_inflight": max_inflight}
     mean_reads = float(sizes.mean())
@@ -223,7 +239,8 @@ def serve_stream(native, ix, reqs, sizes, loads, rng, log, max_batch_reads=65536
         r["offered_requests_per_s"] = rate
         out["loads"].append(r)
         log(f"serving load {f:.2f}: {r['reads_per_s'] / 1e6:.2f} M reads/s, p50 "
-            f"{r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms")
+            f"{r['p50_ms']:.2f} ms, p99 {r['p99_ms']:.2f} ms (wait p99 {r['wait_p99_ms']:.2f}, "
+            f"service p99 {r['service_p99_ms']:.2f} ms)")
     return out
 
 
@@ -437,7 +454,8 @@ def headline_leg(args, torch, native, D):
     if not args.no_serve:
         rng = np.random.default_rng(w.seed)
         reqs, sizes = make_requests(rng, qb, qo, args.serve_requests)
-        sv = serve_stream(native, ix, reqs, sizes, (0.25, 0.5, 0.9), rng, log)
+        sv = serve_stream(native, ix, reqs, sizes, (0.25, 0.5, 0.9), rng, log,
+                          max_batch_reads=args.serve_batch, max_inflight=args.serve_slots)
         sv["workload"] = (f"requests of floor(10^U(0,4)) reads (mean {sizes.mean():.0f}) drawn "
                           "from the 10M reads, Poisson arrivals; latency submit -> results on "
                           "the host")
@@ -500,7 +518,7 @@ def workload2_leg(args, torch, native, D):
         "reads/s")
     rows = kernel_table(tim, kctr, tiles, tiers, 150)
     dom = max(rows, key=lambda x: x["ms"])
-    traffic, src = profiled_traffic("k_classify_workload2_traffic.json", dom["kernel"])
+    traffic, src = profiled_traffic("k_classify_lane16_workload2_traffic.json", dom["kernel"])
     r["roofline"] = roofline_of(dom, traffic, src)
     if not args.no_plain:
         psec, ptim, _, _, pout, ptop = run_steps(torch, handle, ix, d_reads, d_off, n, 0,
@@ -607,6 +625,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--serve-requests", type=int, default=4000,
                     help="requests of the headline's served stream")
+    ap.add_argument("--serve-batch", type=int, default=65536,
+                    help="most reads per served batch (headline stream)")
+    ap.add_argument("--serve-slots", type=int, default=3,
+                    help="served batches on the device at once (headline stream)")
     ap.add_argument("--w4-requests", type=int, default=300)
     ap.add_argument("--plain-requests", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")

@@ -169,7 +169,10 @@ void complete_slot(mh_server *sv, Slot *s) {
       s->ex.h_ctr->top_used)
     e = hipMemcpy(s->h_top, s->d_top, s->ex.h_ctr->top_used * 4, hipMemcpyDeviceToHost);
   const double t = now_s();
-  std::unique_lock<std::mutex> lk(sv->mu);
+  // The batch's requests belong to this slot until they are marked done
+  // under the lock, so the slot-private checks, the pool growth and the
+  // results split below run without it: other slots keep completing and the
+  // scheduler keeps admitting meanwhile.
   int32_t st = (e == hipSuccess && s->launched_ok) ? MH_OK : MH_ERR_HIP;
   bool overflow = false;
   if (st == MH_OK) {
@@ -198,11 +201,7 @@ void complete_slot(mh_server *sv, Slot *s) {
       overflow = false;
     }
   }
-  if (overflow) {
-    // Rare: more tied references than the pools held.  Requeue the batch at
-    // the front; the pools have grown to what it needs.
-    for (auto it = s->batch.rbegin(); it != s->batch.rend(); ++it) sv->pending.push_front(*it);
-  } else {
+  if (!overflow) {
     uint32_t r = 0;
     for (Request *q : s->batch) {
       q->status = st;
@@ -219,9 +218,16 @@ void complete_slot(mh_server *sv, Slot *s) {
         }
       }
       q->done_s = t;
-      q->done = true;
       r += q->n;
     }
+  }
+  std::unique_lock<std::mutex> lk(sv->mu);
+  if (overflow) {
+    // Rare: more tied references than the pools held.  Requeue the batch at
+    // the front; the pools have grown to what it needs.
+    for (auto it = s->batch.rbegin(); it != s->batch.rend(); ++it) sv->pending.push_front(*it);
+  } else {
+    for (Request *q : s->batch) q->done = true;
   }
   sv->inflight_bytes -= s->batch_bytes;
   s->batch.clear();
@@ -366,9 +372,14 @@ mh_status mh_server_wait(mh_server *sv, uint64_t request_id, double timeout_s,
   }
   if (q->top.size() > top_cap && h_top) return MH_ERR_OVERFLOW;
   const mh_status st = q->status;
-  if (h_out && !q->out.empty()) std::memcpy(h_out, q->out.data(), q->out.size() * sizeof(mh_read_result));
-  if (h_top && !q->top.empty()) std::memcpy(h_top, q->top.data(), q->top.size() * 4);
+  // The request leaves the table here; its results are copied out after the
+  // lock is released.
+  std::unique_ptr<Request> own = std::move(it->second);
   sv->reqs.erase(it);
+  lk.unlock();
+  if (h_out && !own->out.empty())
+    std::memcpy(h_out, own->out.data(), own->out.size() * sizeof(mh_read_result));
+  if (h_top && !own->top.empty()) std::memcpy(h_top, own->top.data(), own->top.size() * 4);
   return st;
 }
 
