@@ -2248,7 +2248,8 @@ __device__ __forceinline__ void stats_flush(const Ws &ws, DevCounters *ctr, uint
   TM::sync();
   if (TM::rank() < kNumStats && ws.stats[TM::rank()]) {
     atomicAdd(&ctr->stats[slot & (kStatSlots - 1)][TM::rank()], ws.stats[TM::rank()]);
-    atomicAdd(&ctr->kstats[kid < kKernelRows ? kid : 0][TM::rank()], ws.stats[TM::rank()]);
+    atomicAdd(&ctr->kstats[kid < kKernelRows ? kid : 0][slot & (kKernelSlots - 1)][TM::rank()],
+              ws.stats[TM::rank()]);
   }
 }
 
@@ -3324,7 +3325,9 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
 
 void kernel_counters_from(const DevCounters &d, mh_counters *o, uint32_t n) {
   for (uint32_t k = 0; k < n && k < (uint32_t)kKernelRows; k++) {
-    const unsigned long long *t = d.kstats[k];
+    unsigned long long t[kNumStats] = {};
+    for (int sl = 0; sl < kKernelSlots; sl++)
+      for (int i = 0; i < kNumStats; i++) t[i] += d.kstats[k][sl][i];
     o[k].queries = t[kStrands];
     o[k].probes = t[kProbes];
     o[k].hits = t[kHits];

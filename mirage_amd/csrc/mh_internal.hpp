@@ -77,6 +77,10 @@ enum Stat { kProbes = 0, kHits, kEntries, kSeeds, kCands, kDirect, kExt, kStrand
 constexpr int kKernelRows = 12;
 constexpr int kTierRow0 = 8;
 constexpr int kStatSlots = 256;
+// Slots of each per-kernel row: a launch of one workgroup per query (the
+// plain form) flushes once per query, and one address per statistic would
+// make those atomics wait in turn.
+constexpr int kKernelSlots = 32;
 
 // Device counters, zeroed before each classify call.
 struct DevCounters {
@@ -88,7 +92,7 @@ struct DevCounters {
   unsigned long long tier_head[4];  // queue heads of the overflow tiers
   unsigned long long queue_head[8]; // queue heads of the length buckets
   unsigned long long stats[kStatSlots][8];
-  unsigned long long kstats[kKernelRows][8];  // the same, per kernel (kid)
+  unsigned long long kstats[kKernelRows][kKernelSlots][8];  // the same, per kernel (kid)
 };
 
 struct Outputs {
