@@ -219,6 +219,22 @@ __device__ __forceinline__ uint64_t xor_lane64_any(uint32_t x, uint64_t v) {
     default: return bperm64(src_xor(x), v);
   }
 }
+// The value of lane ^ X with every lane of the wavefront active: DPP inside a
+// row for X = 1, 2, 3 (quad_perm [3,2,1,0]), 4, 7 (row_half_mirror), 8 and
+// 15 (row_mirror); X = 0 stands for any other mask, through ds_bpermute from
+// `src` (the byte address of the source lane).
+template <uint32_t X>
+__device__ __forceinline__ uint32_t xor_mask(uint32_t src, uint32_t v) {
+  if constexpr (X == 3) return dpp<0x1B>(0u, v);
+  else if constexpr (X == 7) return dpp<0x141>(0u, v);
+  else if constexpr (X == 15) return dpp<0x140>(0u, v);
+  else if constexpr (X == 0) return bperm(src, v);
+  else return xor_lane<X>(v);
+}
+template <uint32_t X>
+__device__ __forceinline__ uint64_t xor_mask64(uint32_t src, uint64_t v) {
+  return ((uint64_t)xor_mask<X>(src, (uint32_t)(v >> 32)) << 32) | xor_mask<X>(src, (uint32_t)v);
+}
 
 // The value of lane u of the segment (u the same on every lane of the
 // wavefront): v_readlane for whole wavefronts, else ds_bpermute.
@@ -405,8 +421,8 @@ __device__ __forceinline__ void step_sync(bool local, bool &prev_local) {
   prev_local = local;
 }
 
-template <class TM>
-__device__ void bitonic_sort(uint64_t *key, uint32_t *val, uint32_t n) {
+template <class TM, class K64, class V32>
+__device__ __noinline__ void bitonic_sort_in(K64 *key, V32 *val, uint32_t n) {
   constexpr uint32_t W = TM::kWorkerLanes;
   if (n <= W) {
     TM::sync();
@@ -455,11 +471,135 @@ __device__ void bitonic_sort(uint64_t *key, uint32_t *val, uint32_t n) {
 }
 
 
+__device__ __forceinline__ void cmp_swap(uint64_t &a, uint64_t &b) {
+  const uint64_t lo = a < b ? a : b, hi = a < b ? b : a;
+  a = lo;
+  b = hi;
+}
+
+// The network of bitonic_sort over n keys held in registers by a whole
+// workgroup, K keys a thread: thread t holds positions t*K .. t*K+K-1
+// (n2 = 2^LOGN = next_pow2(n) <= workgroup size * K; positions n and beyond
+// hold +infinity).  Step (LS, LT) of the merge of size 2^LS pairs position e
+// with e ^ M, M = 2^LS - 1 for its first step (LT = LS - 1) and 2^LT for the
+// others; the lower position takes the smaller key.  The steps are unrolled
+// at compile time, so each one's partner rule and lane masks are constants:
+// partners inside a thread swap in registers; partners inside a wavefront
+// come through DPP or ds_bpermute, every key's partner fetched before any is
+// compared; partners in another wavefront (M of 64*K and more: 3 of the 55
+// steps for 1,024 keys on 256 threads) through the key array in LDS between
+// two barriers.
+template <int K, int LS, int LT, class K64>
+__device__ __forceinline__ void net_step(uint64_t (&x)[K], K64 *key, uint32_t e0, uint32_t lane,
+                                         uint32_t n) {
+  constexpr uint32_t LK = K == 1 ? 0 : (K == 2 ? 1 : 2);
+  constexpr uint32_t WK = 64 * K;  // positions per wavefront
+  constexpr bool mirror = LT == LS - 1;
+  constexpr uint32_t stride = 1u << LT;
+  constexpr uint32_t m = mirror ? (2u << LT) - 1 : stride;
+  if constexpr (m < (uint32_t)K) {
+    if constexpr (m == 1) {
+      cmp_swap(x[0], x[1]);
+      if constexpr (K == 4) cmp_swap(x[2], x[3]);
+    } else if constexpr (m == 2) {
+      cmp_swap(x[0], x[2]);
+      cmp_swap(x[1], x[3]);
+    } else {
+      cmp_swap(x[0], x[3]);
+      cmp_swap(x[1], x[2]);
+    }
+  } else if constexpr (m < WK) {
+    // partner lane ^ (m >> LK); its register j ^ (K - 1) in a mirror step
+    // (the low bits of m are all set), else register j
+    constexpr uint32_t lm = m >> LK;
+    constexpr uint32_t X =
+        (lm == 1 || lm == 2 || lm == 3 || lm == 4 || lm == 7 || lm == 8 || lm == 15) ? lm : 0;
+    const uint32_t src = X == 0 ? src_xor(lm) : 0u;
+    const bool lower = (lane & (stride >> LK)) == 0;
+    uint64_t y[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) y[j] = xor_mask64<X>(src, x[mirror ? K - 1 - j : j]);
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      const bool lt = y[j] < x[j];
+      x[j] = (lower == lt) ? y[j] : x[j];
+    }
+  } else {
+    // Positions n and beyond keep +infinity: they are neither stored nor
+    // read, and a position below n whose partner is one of them keeps its
+    // key (the partner is the higher position).
+#pragma unroll
+    for (int j = 0; j < K; j++)
+      if (e0 + j < n) key[e0 + j] = x[j];
+    __syncthreads();
+    const bool lower = (e0 & stride) == 0;
+    uint64_t y[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      const uint32_t p = (e0 + j) ^ m;
+      y[j] = p < n ? key[p] : ~0ull;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      const bool lt = y[j] < x[j];
+      x[j] = (lower == lt) ? y[j] : x[j];
+    }
+  }
+}
+
+// Steps LT .. 0 of the merge of size 2^LS, then the merges up to 2^LOGN.
+template <int K, int LS, int LT, class K64>
+__device__ __forceinline__ void net_merge(uint64_t (&x)[K], K64 *key, uint32_t e0, uint32_t lane,
+                                          uint32_t n) {
+  net_step<K, LS, LT>(x, key, e0, lane, n);
+  if constexpr (LT > 0) net_merge<K, LS, LT - 1>(x, key, e0, lane, n);
+}
+template <int K, int LS, int LOGN, class K64>
+__device__ __forceinline__ void net_sizes(uint64_t (&x)[K], K64 *key, uint32_t e0, uint32_t lane,
+                                          uint32_t n) {
+  net_merge<K, LS, LS - 1>(x, key, e0, lane, n);
+  if constexpr (LS < LOGN) net_sizes<K, LS + 1, LOGN>(x, key, e0, lane, n);
+}
+
+template <int K, int LOGN, class K64>
+__device__ __forceinline__ void reg_sort_keys(K64 *key, uint32_t n) {
+  static_assert(K == 1 || K == 2 || K == 4, "keys per thread");
+  const uint32_t e0 = threadIdx.x * K;
+  const uint32_t lane = lane64();
+  uint64_t x[K];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < K; j++) x[j] = e0 + j < n ? key[e0 + j] : ~0ull;
+  net_sizes<K, 1, LOGN>(x, key, e0, lane, n);
+#pragma unroll
+  for (int j = 0; j < K; j++)
+    if (e0 + j < n) key[e0 + j] = x[j];
+  __syncthreads();
+}
+
+// reg_sort_keys for n2 = 2^lg, lg from LOGN up to LB + 2 (LB = log2 of the
+// workgroup size): K = 1 up to one key a thread, then 2, then 4.  False when
+// lg is beyond.
+template <int LB, int LOGN, class K64>
+__device__ __forceinline__ bool reg_sort_pow2(K64 *key, uint32_t n, uint32_t lg) {
+  if constexpr (LOGN > LB + 2) {
+    return false;
+  } else {
+    if (lg == (uint32_t)LOGN) {
+      reg_sort_keys<(LOGN <= LB ? 1 : (LOGN == LB + 1 ? 2 : 4)), LOGN>(key, n);
+      return true;
+    }
+    return reg_sort_pow2<LB, LOGN + 1>(key, n, lg);
+  }
+}
+
 // Sorts n distinct u64 keys in place: the rank of each key among the first
-// n by shuffles for one worker's worth, otherwise the network of
+// n by shuffles for one worker's worth; in registers (reg_sort_keys) for a
+// whole workgroup with up to 4 keys a thread; otherwise the network of
 // bitonic_sort without a value array.
-template <class TM>
-__device__ void bitonic_sort_keys(uint64_t *key, uint32_t n) {
+template <class TM, class K64>
+__device__ __noinline__ void bitonic_sort_keys_in(K64 *key, uint32_t n) {
   constexpr uint32_t W = TM::kWorkerLanes;
   if (n <= W) {
     TM::sync();
@@ -475,6 +615,13 @@ __device__ void bitonic_sort_keys(uint64_t *key, uint32_t n) {
     return;
   }
   const uint32_t n2 = next_pow2(n);
+  if constexpr (TM::kPerWave == 1) {
+    constexpr int LB = TM::kSize == 64 ? 6 : TM::kSize == 128 ? 7 : TM::kSize == 256 ? 8
+                       : TM::kSize == 512 ? 9 : 10;
+    static_assert((1u << LB) == TM::kSize, "workgroup size");
+    // n > 64 here, so n2 >= 2^7
+    if (reg_sort_pow2<LB, 7>(key, n, 31 - __clz(n2))) return;
+  }
   bool prev_local = false;
   for (uint32_t size = 2; size <= n2; size <<= 1) {
     for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
@@ -496,6 +643,42 @@ __device__ void bitonic_sort_keys(uint64_t *key, uint32_t n) {
   TM::sync();
 }
 
+// The sorts are called, not inlined (inlined, their registers spill in every
+// tile kernel), so a pointer reaches them without its address space and
+// would compile to flat accesses.  Whole-workgroup teams on LDS workspaces
+// get an instantiation on LDS pointers (ds_read / ds_write); HBM workspaces
+// and lane teams the generic one (lane teams sort a few dozen keys, and
+// their tile kernel measured faster that way: 2.44 against 2.52 ms on 2M
+// strands of 150 bp).
+using lds_u64 = __attribute__((address_space(3))) uint64_t;
+using lds_u32 = __attribute__((address_space(3))) uint32_t;
+
+__device__ __forceinline__ bool in_lds(const void *p) {
+  return __builtin_amdgcn_is_shared((const __attribute__((address_space(0))) void *)p);
+}
+
+template <class TM>
+__device__ __forceinline__ void bitonic_sort_keys(uint64_t *key, uint32_t n) {
+  if constexpr (TM::kPerWave == 1) {
+    if (in_lds(key)) {
+      bitonic_sort_keys_in<TM>((lds_u64 *)key, n);
+      return;
+    }
+  }
+  bitonic_sort_keys_in<TM>(key, n);
+}
+
+template <class TM>
+__device__ __forceinline__ void bitonic_sort(uint64_t *key, uint32_t *val, uint32_t n) {
+  if constexpr (TM::kPerWave == 1) {
+    if (in_lds(key)) {
+      bitonic_sort_in<TM>((lds_u64 *)key, (lds_u32 *)val, n);
+      return;
+    }
+  }
+  bitonic_sort_in<TM>(key, val, n);
+}
+
 // Ascending sort of n u64 keys in global memory through an LDS buffer of
 // lcap keys (a power of two), for workspaces that live in HBM.  Same network
 // as bitonic_sort; every run of its steps whose stride is below the chunk
@@ -505,7 +688,8 @@ __device__ void bitonic_sort_keys(uint64_t *key, uint32_t n) {
 // lcap = 2^12 that is 15 global steps and 6 LDS passes in place of 153
 // global steps.  Keys need not be distinct.
 template <class TM>
-__device__ void staged_sort_keys(uint64_t *key, uint32_t n, uint64_t *lds, uint32_t lcap) {
+__device__ void staged_sort_keys(uint64_t *key, uint32_t n, uint64_t *stage, uint32_t lcap) {
+  lds_u64 *lds = (lds_u64 *)stage;  // the buffer is in LDS: ds_read / ds_write
   if (n <= 1) {
     TM::sync();
     return;
@@ -2016,61 +2200,127 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
       }
     }
     // Candidate references in ascending order: rank = index among the
-    // distinct references; k2 becomes rank << 32 | refPos.
-    for (uint32_t t = tid; t < nP; t += BS)
-      ws.t1[t] = (t == 0 || (ws.k2[t - 1] >> 32) != (ws.k2[t] >> 32)) ? 1u : 0u;
-    TM::sync();
-    uint32_t nrank = 0;
-    for (uint32_t base = 0; base < nP; base += BS) {
-      const uint32_t t = base + tid;
-      const uint32_t st = t < nP ? ws.t1[t] : 0u;
-      uint32_t tot;
-      const uint32_t ex = TM::excl_scan(st, ws.red, tot);
-      if (t < nP) {
-        const uint64_t key = ws.k2[t];
-        const uint32_t rank = nrank + ex + st - 1;
-        if (st) ws.rref[rank] = (uint32_t)(key >> 32);
-        ws.k2[t] = ((uint64_t)rank << 32) | (uint32_t)key;
-      }
-      nrank += tot;
-    }
-    nranks = nrank;
-    TM::sync();
-    // Chains: neighbours link when both coordinates step by s
-    // (Analyser.cpp:2351-2353, u32 arithmetic).
-    for (uint32_t t = tid; t < nP; t += BS) {
-      uint32_t st = 1;
-      if (t > 0) {
-        const uint64_t a = ws.k2[t - 1], b = ws.k2[t];
-        st = ((a >> 32) == (b >> 32) && (uint32_t)a == (uint32_t)b - (uint32_t)s &&
-              ws.v2[t - 1] == ws.v2[t] - (uint32_t)s)
-                 ? 0u
-                 : 1u;
-      }
-      ws.t1[t] = st;
-    }
-    TM::sync();
-    uint32_t nch = 0;
-    for (uint32_t base = 0; base < nP; base += BS) {
-      const uint32_t t = base + tid;
-      const uint32_t st = t < nP ? ws.t1[t] : 0u;
-      uint32_t tot;
-      const uint32_t ex = TM::excl_scan(st, ws.red, tot);
-      if (t < nP) ws.t2[t] = nch + ex + st - 1;  // chain id
-      nch += tot;
-    }
-    nchains = nch;
-    TM::sync();
+    // distinct references; k2 becomes rank << 32 | refPos.  Chains:
+    // neighbours link when both coordinates step by s (Analyser.cpp:2351-2353,
+    // u32 arithmetic); cfl[2c] and cfl[2c + 1] are the first and last pair of
+    // chain c, v1[rank] the rank's first chain, t3[rank] zeroed.
     uint32_t *cfl = (uint32_t *)ws.k1;  // chain first/last element
-    for (uint32_t t = tid; t < nP; t += BS) {
-      const uint32_t cid = ws.t2[t];
-      if (ws.t1[t]) cfl[2 * cid] = t;
-      if (t + 1 == nP || ws.t1[t + 1]) cfl[2 * cid + 1] = t;
-      const uint32_t rank = (uint32_t)(ws.k2[t] >> 32);
-      if (t == 0 || (uint32_t)(ws.k2[t - 1] >> 32) != rank) ws.v1[rank] = cid;
+    if constexpr (TM::kPerWave == 1) {
+      // Whole-workgroup teams: each thread takes a contiguous run of `per`
+      // pairs, so that one team scan numbers them all; the scan's barriers
+      // order every read of a neighbour's pair before the pairs are
+      // rewritten (a one-wavefront team syncs instead).
+      const uint32_t per = (nP + BS - 1) / BS;
+      const uint32_t lo = min(nP, tid * per), hi = min(nP, lo + per);
+      {
+        uint32_t cnt = 0;
+        uint32_t prev = lo > 0 ? (uint32_t)(ws.k2[lo - 1] >> 32) : 0u;
+        for (uint32_t t = lo; t < hi; t++) {
+          const uint32_t r = (uint32_t)(ws.k2[t] >> 32);
+          const uint32_t st = (t == 0 || prev != r) ? 1u : 0u;
+          ws.t1[t] = st;
+          cnt += st;
+          prev = r;
+        }
+        uint32_t tot;
+        uint32_t rank = TM::excl_scan(cnt, ws.red, tot) - 1;  // wraps to 0 at a run's start
+        nranks = tot;
+        if constexpr (BS <= 64) TM::sync();
+        for (uint32_t t = lo; t < hi; t++) {
+          const uint64_t key = ws.k2[t];
+          if (ws.t1[t]) {
+            rank++;
+            ws.rref[rank] = (uint32_t)(key >> 32);
+            ws.t3[rank] = 0;
+          }
+          ws.k2[t] = ((uint64_t)rank << 32) | (uint32_t)key;
+        }
+      }
+      TM::sync();
+      {  // chains
+        uint32_t cnt = 0;
+        uint64_t pa = lo > 0 ? ws.k2[lo - 1] : 0ull;
+        uint32_t pq = lo > 0 ? ws.v2[lo - 1] : 0u;
+        for (uint32_t t = lo; t < hi; t++) {
+          const uint64_t b = ws.k2[t];
+          const uint32_t bq = ws.v2[t];
+          const bool link = t > 0 && (pa >> 32) == (b >> 32) &&
+                            (uint32_t)pa == (uint32_t)b - (uint32_t)s && pq == bq - (uint32_t)s;
+          const uint32_t st = link ? 0u : 1u;
+          ws.t1[t] = st;
+          cnt += st;
+          pa = b;
+          pq = bq;
+        }
+        uint32_t tot;
+        uint32_t cid = TM::excl_scan(cnt, ws.red, tot) - 1;
+        nchains = tot;
+        if constexpr (BS <= 64) TM::sync();  // the next run's first flag, written above
+        for (uint32_t t = lo; t < hi; t++) {
+          const uint32_t st = ws.t1[t];
+          cid += st;
+          if (st) cfl[2 * cid] = t;
+          if (t + 1 == nP || ws.t1[t + 1]) cfl[2 * cid + 1] = t;
+          const uint32_t rank = (uint32_t)(ws.k2[t] >> 32);
+          if (t == 0 || (uint32_t)(ws.k2[t - 1] >> 32) != rank) ws.v1[rank] = cid;
+        }
+      }
+      TM::sync();
+    } else {
+      // Lane teams: pair-strided passes, one team scan per kSize pairs
+      // (fewer registers live than the contiguous runs above).
+      for (uint32_t t = tid; t < nP; t += BS)
+        ws.t1[t] = (t == 0 || (ws.k2[t - 1] >> 32) != (ws.k2[t] >> 32)) ? 1u : 0u;
+      TM::sync();
+      uint32_t nrank = 0;
+      for (uint32_t base = 0; base < nP; base += BS) {
+        const uint32_t t = base + tid;
+        const uint32_t st = t < nP ? ws.t1[t] : 0u;
+        uint32_t tot;
+        const uint32_t ex = TM::excl_scan(st, ws.red, tot);
+        if (t < nP) {
+          const uint64_t key = ws.k2[t];
+          const uint32_t rank = nrank + ex + st - 1;
+          if (st) ws.rref[rank] = (uint32_t)(key >> 32);
+          ws.k2[t] = ((uint64_t)rank << 32) | (uint32_t)key;
+        }
+        nrank += tot;
+      }
+      nranks = nrank;
+      TM::sync();
+      for (uint32_t t = tid; t < nP; t += BS) {  // chains
+        uint32_t st = 1;
+        if (t > 0) {
+          const uint64_t a = ws.k2[t - 1], b = ws.k2[t];
+          st = ((a >> 32) == (b >> 32) && (uint32_t)a == (uint32_t)b - (uint32_t)s &&
+                ws.v2[t - 1] == ws.v2[t] - (uint32_t)s)
+                   ? 0u
+                   : 1u;
+        }
+        ws.t1[t] = st;
+      }
+      TM::sync();
+      uint32_t nch = 0;
+      for (uint32_t base = 0; base < nP; base += BS) {
+        const uint32_t t = base + tid;
+        const uint32_t st = t < nP ? ws.t1[t] : 0u;
+        uint32_t tot;
+        const uint32_t ex = TM::excl_scan(st, ws.red, tot);
+        if (t < nP) ws.t2[t] = nch + ex + st - 1;  // chain id
+        nch += tot;
+      }
+      nchains = nch;
+      TM::sync();
+      for (uint32_t t = tid; t < nP; t += BS) {
+        const uint32_t cid = ws.t2[t];
+        if (ws.t1[t]) cfl[2 * cid] = t;
+        if (t + 1 == nP || ws.t1[t + 1]) cfl[2 * cid + 1] = t;
+        const uint32_t rank = (uint32_t)(ws.k2[t] >> 32);
+        if (t == 0 || (uint32_t)(ws.k2[t - 1] >> 32) != rank) ws.v1[rank] = cid;
+      }
+      for (uint32_t r = tid; r < nranks; r += BS) ws.t3[r] = 0;
+      TM::sync();
     }
-    for (uint32_t r = tid; r < nranks; r += BS) ws.t3[r] = 0;
-    TM::sync();
     for (uint32_t cid = tid; cid < nchains; cid += BS) {
       const uint32_t f = cfl[2 * cid], l = cfl[2 * cid + 1];
       atomicMax(&ws.t3[(uint32_t)(ws.k2[f] >> 32)], l - f + 1);
