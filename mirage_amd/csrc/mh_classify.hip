@@ -1105,7 +1105,14 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
   const uint32_t *cfl = (const uint32_t *)ws.k1;
   float *rel_of = (float *)ws.t2;
   uint32_t *valid_of = ws.t1;
-  const uint32_t bw = cap.qcap / 32 + 1, nw = (m + 31) >> 5;
+  const uint32_t nw = (m + 31) >> 5, bw = nw + 1;  // row bitmap words for this read
+  // The slot region (cap.nslot slots whose bitmaps fit reads of cap.qcap
+  // bases) holds more slots for a shorter read: up to one per thread, so that
+  // fewer rounds score a strand's references (a 150 bp read gets 69 in the
+  // 1 kb second-chance tier instead of 31).
+  const uint32_t region = cap.nslot * (uint32_t)(sizeof(RefSlot) + (cap.qcap / 32 + 1) * 4);
+  const uint32_t nslot = min(BS, region / (uint32_t)(sizeof(RefSlot) + bw * 4));
+  uint32_t *const slot_bm = (uint32_t *)(ws.slots + nslot);
   seeds_total = 0;
   seeded = 0;
   unsigned long long seeds_part = 0;  // this thread's slots' seeds
@@ -1126,8 +1133,8 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
       ro = ix.ref_off[R];
       rlen = ix.ref_off[R + 1] - ro;
     }
-    for (uint32_t g = 0; g < tot; g += cap.nslot) {
-      const uint32_t nb = min(cap.nslot, tot - g);
+    for (uint32_t g = 0; g < tot; g += nslot) {
+      const uint32_t nb = min(nslot, tot - g);
       TM::sync();
       if (f && idx >= g && idx < g + nb) {
         RefSlot &sl = ws.slots[idx - g];
@@ -1146,7 +1153,7 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
         sl.nchain = c1 - c0;
         sl.nseed = c1 - c0 + nvirt;
       }
-      for (uint32_t w = tid; w < nb * bw; w += BS) ws.slot_bm[w] = 0;
+      for (uint32_t w = tid; w < nb * bw; w += BS) slot_bm[w] = 0;
       TM::sync();
       // each slot's first item: a scan of the seed counts (nb <= BS)
       uint32_t nitems;
@@ -1209,7 +1216,7 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
                       (unsigned long long)wt * ((unsigned long long)tl * (tl + 1) / 2));
             atomicAdd(&ws.slots[b].acc[1], (unsigned long long)wt * tl);
           }
-          set_bits<TM>(ws.slot_bm + b * bw, rs - h, tl);
+          set_bits<TM>(slot_bm + b * bw, rs - h, tl);
         }
       }
       if (lane == 0 && ncmp && ws.stats) atomicAdd(&ws.stats[kExt], ncmp);
@@ -1218,7 +1225,7 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
       // the relative score waits for the pass after the last round (the table
       // lookups of every scored reference in flight together)
       for (uint32_t b = tid; b < nb; b += BS) {
-        const uint32_t *bm = ws.slot_bm + b * bw;
+        const uint32_t *bm = slot_bm + b * bw;
         uint32_t hits = 0;
         for (uint32_t w = 0; w < nw; w++) hits += __popc(bm[w]);
         if (!c.out.cands) {
