@@ -421,6 +421,189 @@ __device__ __forceinline__ void step_sync(bool local, bool &prev_local) {
   prev_local = local;
 }
 
+// Elements of the register networks: a u64 key, or a (u64 key, u32 value)
+// pair ordered by key, then value.
+struct KeyVal {
+  uint64_t k;
+  uint32_t v;
+};
+__device__ __forceinline__ bool elem_less(uint64_t a, uint64_t b) { return a < b; }
+__device__ __forceinline__ bool elem_less(const KeyVal &a, const KeyVal &b) {
+  return pair_less(a.k, a.v, b.k, b.v);
+}
+template <uint32_t X>
+__device__ __forceinline__ uint64_t elem_xor(uint32_t src, uint64_t a) {
+  return xor_mask64<X>(src, a);
+}
+template <uint32_t X>
+__device__ __forceinline__ KeyVal elem_xor(uint32_t src, const KeyVal &a) {
+  return KeyVal{xor_mask64<X>(src, a.k), xor_mask<X>(src, a.v)};
+}
+// c ? a : b, field by field (a select of whole structs would go through
+// their addresses and keep the network's arrays out of registers)
+__device__ __forceinline__ uint64_t elem_sel(bool c, uint64_t a, uint64_t b) { return c ? a : b; }
+__device__ __forceinline__ KeyVal elem_sel(bool c, const KeyVal &a, const KeyVal &b) {
+  return KeyVal{c ? a.k : b.k, c ? a.v : b.v};
+}
+template <class El>
+__device__ __forceinline__ void cmp_swap(El &a, El &b) {
+  const bool sw = elem_less(b, a);
+  const El lo = elem_sel(sw, b, a), hi = elem_sel(sw, a, b);
+  a = lo;
+  b = hi;
+}
+
+// The arrays a network loads from and stores to: keys alone, or keys and
+// values side by side.
+template <class K64>
+struct KeyArr {
+  using El = uint64_t;
+  K64 *key;
+  __device__ __forceinline__ El inf() const { return ~0ull; }
+  __device__ __forceinline__ El get(uint32_t i) const { return key[i]; }
+  __device__ __forceinline__ void put(uint32_t i, const El &x) const { key[i] = x; }
+};
+template <class K64, class V32>
+struct PairArr {
+  using El = KeyVal;
+  K64 *key;
+  V32 *val;
+  __device__ __forceinline__ El inf() const { return KeyVal{~0ull, ~0u}; }
+  __device__ __forceinline__ El get(uint32_t i) const { return KeyVal{key[i], val[i]}; }
+  __device__ __forceinline__ void put(uint32_t i, const El &x) const {
+    key[i] = x.k;
+    val[i] = x.v;
+  }
+};
+
+// The network of bitonic_sort over n elements held in registers by a whole
+// workgroup, K a thread: thread t holds positions t*K .. t*K+K-1 (n2 = 2^LOGN
+// = next_pow2(n) <= workgroup size * K; positions n and beyond hold
+// +infinity).  Step (LS, LT) of the merge of size 2^LS pairs position e with
+// e ^ M, M = 2^LS - 1 for its first step (LT = LS - 1) and 2^LT for the
+// others; the lower position takes the smaller element.  The steps are
+// unrolled at compile time, so each one's partner rule and lane masks are
+// constants: partners inside a thread swap in registers; partners inside a
+// wavefront come through DPP or ds_bpermute, every element's partner fetched
+// before any is compared; partners in another wavefront (M of 64*K and more:
+// 3 of the 55 steps for 1,024 keys on 256 threads) through the arrays in LDS
+// between two barriers.
+template <int K, int LS, int LT, class A>
+__device__ __forceinline__ void net_step(typename A::El (&x)[K], const A &arr, uint32_t e0,
+                                         uint32_t lane, uint32_t n) {
+  using El = typename A::El;
+  constexpr uint32_t LK = K == 1 ? 0 : (K == 2 ? 1 : 2);
+  constexpr uint32_t WK = 64 * K;  // positions per wavefront
+  constexpr bool mirror = LT == LS - 1;
+  constexpr uint32_t stride = 1u << LT;
+  constexpr uint32_t m = mirror ? (2u << LT) - 1 : stride;
+  if constexpr (m < (uint32_t)K) {
+    if constexpr (m == 1) {
+      cmp_swap(x[0], x[1]);
+      if constexpr (K == 4) cmp_swap(x[2], x[3]);
+    } else if constexpr (m == 2) {
+      cmp_swap(x[0], x[2]);
+      cmp_swap(x[1], x[3]);
+    } else {
+      cmp_swap(x[0], x[3]);
+      cmp_swap(x[1], x[2]);
+    }
+  } else if constexpr (m < WK) {
+    // partner lane ^ (m >> LK); its register j ^ (K - 1) in a mirror step
+    // (the low bits of m are all set), else register j
+    constexpr uint32_t lm = m >> LK;
+    constexpr uint32_t X =
+        (lm == 1 || lm == 2 || lm == 3 || lm == 4 || lm == 7 || lm == 8 || lm == 15) ? lm : 0;
+    const uint32_t src = X == 0 ? src_xor(lm) : 0u;
+    const bool lower = (lane & (stride >> LK)) == 0;
+    El y[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) y[j] = elem_xor<X>(src, x[mirror ? K - 1 - j : j]);
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      const bool lt = elem_less(y[j], x[j]);
+      x[j] = elem_sel(lower == lt, y[j], x[j]);
+    }
+  } else {
+    // Positions n and beyond keep +infinity: they are neither stored nor
+    // read, and a position below n whose partner is one of them keeps its
+    // element (the partner is the higher position).
+#pragma unroll
+    for (int j = 0; j < K; j++)
+      if (e0 + j < n) arr.put(e0 + j, x[j]);
+    __syncthreads();
+    const bool lower = (e0 & stride) == 0;
+    El y[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      const uint32_t p = (e0 + j) ^ m;
+      if (p < n) y[j] = arr.get(p);
+      else y[j] = arr.inf();
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      const bool lt = elem_less(y[j], x[j]);
+      x[j] = elem_sel(lower == lt, y[j], x[j]);
+    }
+  }
+}
+
+// Steps LT .. 0 of the merge of size 2^LS, then the merges up to 2^LOGN.
+template <int K, int LS, int LT, class A>
+__device__ __forceinline__ void net_merge(typename A::El (&x)[K], const A &arr, uint32_t e0,
+                                          uint32_t lane, uint32_t n) {
+  net_step<K, LS, LT>(x, arr, e0, lane, n);
+  if constexpr (LT > 0) net_merge<K, LS, LT - 1>(x, arr, e0, lane, n);
+}
+template <int K, int LS, int LOGN, class A>
+__device__ __forceinline__ void net_sizes(typename A::El (&x)[K], const A &arr, uint32_t e0,
+                                          uint32_t lane, uint32_t n) {
+  net_merge<K, LS, LS - 1>(x, arr, e0, lane, n);
+  if constexpr (LS < LOGN) net_sizes<K, LS + 1, LOGN>(x, arr, e0, lane, n);
+}
+
+template <int K, int LOGN, class A>
+__device__ __forceinline__ void reg_sort(const A &arr, uint32_t n) {
+  static_assert(K == 1 || K == 2 || K == 4, "elements per thread");
+  const uint32_t e0 = threadIdx.x * K;
+  const uint32_t lane = lane64();
+  typename A::El x[K];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    if (e0 + j < n) x[j] = arr.get(e0 + j);
+    else x[j] = arr.inf();
+  }
+  net_sizes<K, 1, LOGN>(x, arr, e0, lane, n);
+#pragma unroll
+  for (int j = 0; j < K; j++)
+    if (e0 + j < n) arr.put(e0 + j, x[j]);
+  __syncthreads();
+}
+
+// reg_sort for n2 = 2^lg, lg from LOGN up to LB + 2 (LB = log2 of the
+// workgroup size): K = 1 up to one element a thread, then 2, then 4.  False
+// when lg is beyond.
+template <int LB, int LOGN, class A>
+__device__ __forceinline__ bool reg_sort_pow2(const A &arr, uint32_t n, uint32_t lg) {
+  if constexpr (LOGN > LB + 2) {
+    return false;
+  } else {
+    if (lg == (uint32_t)LOGN) {
+      reg_sort<(LOGN <= LB ? 1 : (LOGN == LB + 1 ? 2 : 4)), LOGN>(arr, n);
+      return true;
+    }
+    return reg_sort_pow2<LB, LOGN + 1>(arr, n, lg);
+  }
+}
+
+// log2 of a whole-workgroup team's size
+template <class TM>
+constexpr int team_log2() {
+  return TM::kSize == 64 ? 6 : TM::kSize == 128 ? 7 : TM::kSize == 256 ? 8 : TM::kSize == 512 ? 9 : 10;
+}
+
 template <class TM, class K64, class V32>
 __device__ __noinline__ void bitonic_sort_in(K64 *key, V32 *val, uint32_t n) {
   constexpr uint32_t W = TM::kWorkerLanes;
@@ -446,6 +629,11 @@ __device__ __noinline__ void bitonic_sort_in(K64 *key, V32 *val, uint32_t n) {
     return;
   }
   const uint32_t n2 = next_pow2(n);
+  if constexpr (TM::kPerWave == 1) {
+    static_assert((1 << team_log2<TM>()) == (int)TM::kSize, "workgroup size");
+    // n > 64 here, so n2 >= 2^7
+    if (reg_sort_pow2<team_log2<TM>(), 7>(PairArr<K64, V32>{key, val}, n, 31 - __clz(n2))) return;
+  }
   bool prev_local = false;
   for (uint32_t size = 2; size <= n2; size <<= 1) {
     for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
@@ -471,129 +659,6 @@ __device__ __noinline__ void bitonic_sort_in(K64 *key, V32 *val, uint32_t n) {
 }
 
 
-__device__ __forceinline__ void cmp_swap(uint64_t &a, uint64_t &b) {
-  const uint64_t lo = a < b ? a : b, hi = a < b ? b : a;
-  a = lo;
-  b = hi;
-}
-
-// The network of bitonic_sort over n keys held in registers by a whole
-// workgroup, K keys a thread: thread t holds positions t*K .. t*K+K-1
-// (n2 = 2^LOGN = next_pow2(n) <= workgroup size * K; positions n and beyond
-// hold +infinity).  Step (LS, LT) of the merge of size 2^LS pairs position e
-// with e ^ M, M = 2^LS - 1 for its first step (LT = LS - 1) and 2^LT for the
-// others; the lower position takes the smaller key.  The steps are unrolled
-// at compile time, so each one's partner rule and lane masks are constants:
-// partners inside a thread swap in registers; partners inside a wavefront
-// come through DPP or ds_bpermute, every key's partner fetched before any is
-// compared; partners in another wavefront (M of 64*K and more: 3 of the 55
-// steps for 1,024 keys on 256 threads) through the key array in LDS between
-// two barriers.
-template <int K, int LS, int LT, class K64>
-__device__ __forceinline__ void net_step(uint64_t (&x)[K], K64 *key, uint32_t e0, uint32_t lane,
-                                         uint32_t n) {
-  constexpr uint32_t LK = K == 1 ? 0 : (K == 2 ? 1 : 2);
-  constexpr uint32_t WK = 64 * K;  // positions per wavefront
-  constexpr bool mirror = LT == LS - 1;
-  constexpr uint32_t stride = 1u << LT;
-  constexpr uint32_t m = mirror ? (2u << LT) - 1 : stride;
-  if constexpr (m < (uint32_t)K) {
-    if constexpr (m == 1) {
-      cmp_swap(x[0], x[1]);
-      if constexpr (K == 4) cmp_swap(x[2], x[3]);
-    } else if constexpr (m == 2) {
-      cmp_swap(x[0], x[2]);
-      cmp_swap(x[1], x[3]);
-    } else {
-      cmp_swap(x[0], x[3]);
-      cmp_swap(x[1], x[2]);
-    }
-  } else if constexpr (m < WK) {
-    // partner lane ^ (m >> LK); its register j ^ (K - 1) in a mirror step
-    // (the low bits of m are all set), else register j
-    constexpr uint32_t lm = m >> LK;
-    constexpr uint32_t X =
-        (lm == 1 || lm == 2 || lm == 3 || lm == 4 || lm == 7 || lm == 8 || lm == 15) ? lm : 0;
-    const uint32_t src = X == 0 ? src_xor(lm) : 0u;
-    const bool lower = (lane & (stride >> LK)) == 0;
-    uint64_t y[K];
-#pragma unroll
-    for (int j = 0; j < K; j++) y[j] = xor_mask64<X>(src, x[mirror ? K - 1 - j : j]);
-#pragma unroll
-    for (int j = 0; j < K; j++) {
-      const bool lt = y[j] < x[j];
-      x[j] = (lower == lt) ? y[j] : x[j];
-    }
-  } else {
-    // Positions n and beyond keep +infinity: they are neither stored nor
-    // read, and a position below n whose partner is one of them keeps its
-    // key (the partner is the higher position).
-#pragma unroll
-    for (int j = 0; j < K; j++)
-      if (e0 + j < n) key[e0 + j] = x[j];
-    __syncthreads();
-    const bool lower = (e0 & stride) == 0;
-    uint64_t y[K];
-#pragma unroll
-    for (int j = 0; j < K; j++) {
-      const uint32_t p = (e0 + j) ^ m;
-      y[j] = p < n ? key[p] : ~0ull;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < K; j++) {
-      const bool lt = y[j] < x[j];
-      x[j] = (lower == lt) ? y[j] : x[j];
-    }
-  }
-}
-
-// Steps LT .. 0 of the merge of size 2^LS, then the merges up to 2^LOGN.
-template <int K, int LS, int LT, class K64>
-__device__ __forceinline__ void net_merge(uint64_t (&x)[K], K64 *key, uint32_t e0, uint32_t lane,
-                                          uint32_t n) {
-  net_step<K, LS, LT>(x, key, e0, lane, n);
-  if constexpr (LT > 0) net_merge<K, LS, LT - 1>(x, key, e0, lane, n);
-}
-template <int K, int LS, int LOGN, class K64>
-__device__ __forceinline__ void net_sizes(uint64_t (&x)[K], K64 *key, uint32_t e0, uint32_t lane,
-                                          uint32_t n) {
-  net_merge<K, LS, LS - 1>(x, key, e0, lane, n);
-  if constexpr (LS < LOGN) net_sizes<K, LS + 1, LOGN>(x, key, e0, lane, n);
-}
-
-template <int K, int LOGN, class K64>
-__device__ __forceinline__ void reg_sort_keys(K64 *key, uint32_t n) {
-  static_assert(K == 1 || K == 2 || K == 4, "keys per thread");
-  const uint32_t e0 = threadIdx.x * K;
-  const uint32_t lane = lane64();
-  uint64_t x[K];
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < K; j++) x[j] = e0 + j < n ? key[e0 + j] : ~0ull;
-  net_sizes<K, 1, LOGN>(x, key, e0, lane, n);
-#pragma unroll
-  for (int j = 0; j < K; j++)
-    if (e0 + j < n) key[e0 + j] = x[j];
-  __syncthreads();
-}
-
-// reg_sort_keys for n2 = 2^lg, lg from LOGN up to LB + 2 (LB = log2 of the
-// workgroup size): K = 1 up to one key a thread, then 2, then 4.  False when
-// lg is beyond.
-template <int LB, int LOGN, class K64>
-__device__ __forceinline__ bool reg_sort_pow2(K64 *key, uint32_t n, uint32_t lg) {
-  if constexpr (LOGN > LB + 2) {
-    return false;
-  } else {
-    if (lg == (uint32_t)LOGN) {
-      reg_sort_keys<(LOGN <= LB ? 1 : (LOGN == LB + 1 ? 2 : 4)), LOGN>(key, n);
-      return true;
-    }
-    return reg_sort_pow2<LB, LOGN + 1>(key, n, lg);
-  }
-}
-
 // Sorts n distinct u64 keys in place: the rank of each key among the first
 // n by shuffles for one worker's worth; in registers (reg_sort_keys) for a
 // whole workgroup with up to 4 keys a thread; otherwise the network of
@@ -616,11 +681,9 @@ __device__ __noinline__ void bitonic_sort_keys_in(K64 *key, uint32_t n) {
   }
   const uint32_t n2 = next_pow2(n);
   if constexpr (TM::kPerWave == 1) {
-    constexpr int LB = TM::kSize == 64 ? 6 : TM::kSize == 128 ? 7 : TM::kSize == 256 ? 8
-                       : TM::kSize == 512 ? 9 : 10;
-    static_assert((1u << LB) == TM::kSize, "workgroup size");
+    static_assert((1 << team_log2<TM>()) == (int)TM::kSize, "workgroup size");
     // n > 64 here, so n2 >= 2^7
-    if (reg_sort_pow2<LB, 7>(key, n, 31 - __clz(n2))) return;
+    if (reg_sort_pow2<team_log2<TM>(), 7>(KeyArr<K64>{key}, n, 31 - __clz(n2))) return;
   }
   bool prev_local = false;
   for (uint32_t size = 2; size <= n2; size <<= 1) {
@@ -1511,22 +1574,33 @@ __device__ uint32_t filter_masked(const Ws &ws, uint32_t E, uint32_t cap, uint32
   nP = 0;
   if (maxrun < 2) return maxrun;  // cut 1 keeps every pair, and none links
   const uint32_t cut = maxrun - 1;
-  uint32_t nkeep = 0;
-  for (uint32_t base = 0; base < E; base += BS) {
-    const uint32_t e = base + tid;
-    uint32_t keep = 0;
-    uint64_t pk = 0;
-    if (e < E) {
-      const uint64_t k = ws.k1[e];
-      keep = ws.v2[e] >= cut ? 1u : 0u;
-      pk = ((k >> 32) << 38) | ((uint64_t)ws.v1[e] << 6) | ((uint32_t)k & 63);
-    }
+  auto pack = [&](uint32_t e) {
+    const uint64_t k = ws.k1[e];
+    return ((k >> 32) << 38) | ((uint64_t)ws.v1[e] << 6) | ((uint32_t)k & 63);
+  };
+  if constexpr (TM::kPerWave == 1 && BS > 64) {
+    // contiguous runs of entries, one team scan (two barriers) for all of them
+    const uint32_t per = (E + BS - 1) / BS;
+    const uint32_t lo = min(E, tid * per), hi = min(E, lo + per);
+    uint32_t cnt = 0;
+    for (uint32_t e = lo; e < hi; e++) cnt += ws.v2[e] >= cut ? 1u : 0u;
     uint32_t tot;
-    const uint32_t ex = TM::excl_scan(keep, ws.red, tot);
-    if (keep) ws.k2[nkeep + ex] = pk;
-    nkeep += tot;
+    uint32_t at = TM::excl_scan(cnt, ws.red, tot);
+    for (uint32_t e = lo; e < hi; e++)
+      if (ws.v2[e] >= cut) ws.k2[at++] = pack(e);
+    nP = tot;
+  } else {
+    uint32_t nkeep = 0;
+    for (uint32_t base = 0; base < E; base += BS) {
+      const uint32_t e = base + tid;
+      const uint32_t keep = (e < E && ws.v2[e] >= cut) ? 1u : 0u;
+      uint32_t tot;
+      const uint32_t ex = TM::excl_scan(keep, ws.red, tot);
+      if (keep) ws.k2[nkeep + ex] = pack(e);
+      nkeep += tot;
+    }
+    nP = nkeep;
   }
-  nP = nkeep;
   TM::sync();
   return maxrun;
 }
