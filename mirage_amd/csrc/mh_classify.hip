@@ -606,6 +606,15 @@ constexpr int team_log2() {
 
 template <class TM, class K64, class V32>
 __device__ __noinline__ void bitonic_sort_in(K64 *key, V32 *val, uint32_t n) {
+  if constexpr (TM::kPerWave == 1) {
+    // whole-workgroup teams: the register network from two pairs up (one
+    // wavefront: from 65, as for keys)
+    static_assert((1 << team_log2<TM>()) == (int)TM::kSize, "workgroup size");
+    constexpr int LO = TM::kSize == 64 ? 7 : 1;
+    if (n >= 2 && reg_sort_pow2<team_log2<TM>(), LO>(PairArr<K64, V32>{key, val}, n,
+                                                     31 - __clz(next_pow2(n))))
+      return;
+  }
   constexpr uint32_t W = TM::kWorkerLanes;
   if (n <= W) {
     TM::sync();
@@ -629,11 +638,6 @@ __device__ __noinline__ void bitonic_sort_in(K64 *key, V32 *val, uint32_t n) {
     return;
   }
   const uint32_t n2 = next_pow2(n);
-  if constexpr (TM::kPerWave == 1) {
-    static_assert((1 << team_log2<TM>()) == (int)TM::kSize, "workgroup size");
-    // n > 64 here, so n2 >= 2^7
-    if (reg_sort_pow2<team_log2<TM>(), 7>(PairArr<K64, V32>{key, val}, n, 31 - __clz(n2))) return;
-  }
   bool prev_local = false;
   for (uint32_t size = 2; size <= n2; size <<= 1) {
     for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
@@ -665,6 +669,15 @@ __device__ __noinline__ void bitonic_sort_in(K64 *key, V32 *val, uint32_t n) {
 // bitonic_sort without a value array.
 template <class TM, class K64>
 __device__ __noinline__ void bitonic_sort_keys_in(K64 *key, uint32_t n) {
+  if constexpr (TM::kPerWave == 1) {
+    // Whole-workgroup teams: the register network from two keys up.  A
+    // one-wavefront team ranks up to 64 keys by shuffles (measured faster:
+    // it has no other wavefront to leave idle).
+    static_assert((1 << team_log2<TM>()) == (int)TM::kSize, "workgroup size");
+    constexpr int LO = TM::kSize == 64 ? 7 : 1;
+    if (n >= 2 && reg_sort_pow2<team_log2<TM>(), LO>(KeyArr<K64>{key}, n, 31 - __clz(next_pow2(n))))
+      return;
+  }
   constexpr uint32_t W = TM::kWorkerLanes;
   if (n <= W) {
     TM::sync();
@@ -680,11 +693,6 @@ __device__ __noinline__ void bitonic_sort_keys_in(K64 *key, uint32_t n) {
     return;
   }
   const uint32_t n2 = next_pow2(n);
-  if constexpr (TM::kPerWave == 1) {
-    static_assert((1 << team_log2<TM>()) == (int)TM::kSize, "workgroup size");
-    // n > 64 here, so n2 >= 2^7
-    if (reg_sort_pow2<team_log2<TM>(), 7>(KeyArr<K64>{key}, n, 31 - __clz(n2))) return;
-  }
   bool prev_local = false;
   for (uint32_t size = 2; size <= n2; size <<= 1) {
     for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
