@@ -3517,6 +3517,10 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, bw, 64u);
   if (nq) {
     const unsigned gr = (unsigned)((n + 255) / 256);
+    // Probe records of the first length tile's strands, also read by the LDS
+    // overflow tiers for the strands that tile passes on (none without
+    // MH_CLASSIFY_PREPROBE).
+    Ctx crec = c;
     if (!batched) {
       // Plain form: one launch per batch, one workgroup per query, with the
       // single tile that covers the longest read of the batch.
@@ -3581,6 +3585,10 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
         c0.rec = rec;
         c0.rec_stride = rec_stride;
         c0.rec_maxlen = maxlen;
+        // a tier's strand has a record when it was in the first tile's bin
+        crec.rec = rec;
+        crec.rec_stride = rec_stride;
+        crec.rec_maxlen = std::min<uint32_t>(maxlen, tiles[0].max_len);
       }
       // Long buckets first so that their tail overlaps with the short ones.
       if (timed) hipEventRecord(x.ev[9], s);
@@ -3612,7 +3620,7 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
     const std::vector<mh_tile> lds_tiers = tier_tiles(h, a.ix);  // lds_ecap_max: tier 0's cap
     for (int t = 0; t < 2; t++) {
       const Caps tc = tile_caps(lds_tiers[t], a.ix.s, h->lds_per_block);
-      Ctx ct = c;
+      Ctx ct = crec;
       ct.kid = kTierRow0 + (uint32_t)t;
       ct.out.overflow = list[(t + 1) & 1];
       ct.out.overflow_cnt = &d->tier_n[t + 1];

@@ -34,11 +34,14 @@ def repetitive():
     return w, rb, ro, qb, qo, (oout, otop, ocands)
 
 
-@pytest.mark.parametrize("mode", ["defaults", "hbm_tiers", "plain"])
+@pytest.mark.parametrize("mode", ["defaults", "hbm_tiers", "plain", "preprobe",
+                                  "preprobe_hbm_tiers"])
 def test_repetitive_index_matches_oracle(handle, repetitive, mode):
     """Every read and candidate equals the oracle's, with the index-sized
     default tiles, with LDS tiers too small for most strands (so they finish
-    in the HBM-workspace tiers), and in the plain form."""
+    in the HBM-workspace tiers), and in the plain form; with the probe pass,
+    whose records the overflow tiers read for the strands the first tile
+    passes on."""
     w, rb, ro, qb, qo, (oout, otop, ocands) = repetitive
     assert ro[-1] >= 2_000_000_000 * 0.99
     ix = handle.build_index(rb, ro, w.threshold)
@@ -46,8 +49,10 @@ def test_repetitive_index_matches_oracle(handle, repetitive, mode):
     epp = info["n_entries"] / info["n_buckets"]
     assert epp > 7
     flags = 0 if mode == "plain" else native.MH_CLASSIFY_BATCHED
+    if mode.startswith("preprobe"):
+        flags |= native.MH_CLASSIFY_PREPROBE
     try:
-        if mode == "hbm_tiers":
+        if mode.endswith("hbm_tiers"):
             handle.set_tiles([(200, 16, 32, 0), (handle.max_read_len, 64, 64, 0)])
             handle.set_overflow_tiles([(256, 64, 32, 0), (handle.max_read_len, 64, 32, 0)])
         out, top, cands = ix.classify_host(qb, qo, flags=flags, want_cands=True)
@@ -61,8 +66,10 @@ def test_repetitive_index_matches_oracle(handle, repetitive, mode):
     compare_reads(nout, ntop, oout, otop, f"repetitive {mode}, no candidate output")
     compare_cands(cands, ocands, f"repetitive {mode}")
     assert max(len(t) for t in native.tie_sets(out, top)) > 16  # segment ties
-    if mode == "hbm_tiers":
+    if mode.endswith("hbm_tiers"):
         assert rows[10]["queries"] > 0.5 * 2 * (len(qo) - 1)  # most strands in HBM workspaces
+    if mode == "preprobe":
+        assert rows[8]["queries"] > 0.05 * 2 * (len(qo) - 1)  # the second-chance tier reads records
 
 
 def test_mixed_lengths_on_the_full_index(handle):
