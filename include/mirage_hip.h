@@ -57,7 +57,8 @@ typedef struct mh_index mh_index;
 #define MH_CLASSIFY_GRAPH 16u      /* batched form replayed from a captured hipGraph;
                                       captured again when any argument changes */
 #define MH_CLASSIFY_PREPROBE 32u   /* batched form: hash and look up the probes of the
-                                      shortest length tile in a separate pass */
+                                      shortest length tile in a separate pass, whose
+                                      records the LDS overflow tiers also read */
 
 /* Per-read result after the strand merge (MirAge/Analyser.cpp:592-611).
  * Top-tie reference ids (ascending) are top_pool[top_off, top_off+n_top).
