@@ -2903,20 +2903,39 @@ __global__ void k_merge_count(const QueryOut *qo, uint32_t n_reads, mh_read_resu
   }
 }
 
+// One read per lane; a tie set of up to kTopInline references is copied by
+// its lane, a longer one (from the tie pool) by the whole wavefront in turn,
+// in consecutive words.  The grid is a whole number of wavefronts.
 __global__ void k_merge_write(const QueryOut *qo, uint32_t n_reads, const uint32_t *off,
                               const uint32_t *ties, uint64_t tie_cap, mh_read_result *out,
                               uint32_t *top, uint64_t top_cap, DevCounters *ctr) {
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n_reads;
-       r += gridDim.x * blockDim.x) {
-    const uint32_t n = out[r].n_top;
-    const uint32_t o = off[r];
-    out[r].top_off = o;
-    if (!n || (uint64_t)o + n > top_cap) continue;
-    const QueryOut &w = qo[2 * r + out[r].strand];
-    if (n <= (uint32_t)kTopInline) {
-      for (uint32_t i = 0; i < n; i++) top[o + i] = w.top_ref[i];
-    } else if ((uint64_t)w.tie_off + n <= tie_cap) {
-      for (uint32_t i = 0; i < n; i++) top[o + i] = ties[w.tie_off + i];
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; base < n_reads;
+       base += gridDim.x * blockDim.x) {
+    const uint32_t r = base + lane;
+    uint32_t n = 0, o = 0, src = 0;  // a pooled set: ties[src, src + n)
+    if (r < n_reads) {
+      n = out[r].n_top;
+      o = off[r];
+      out[r].top_off = o;
+      if (n && (uint64_t)o + n <= top_cap) {
+        const QueryOut &w = qo[2 * r + out[r].strand];
+        if (n <= (uint32_t)kTopInline) {
+          for (uint32_t i = 0; i < n; i++) top[o + i] = w.top_ref[i];
+          n = 0;
+        } else if ((uint64_t)w.tie_off + n <= tie_cap) {
+          src = w.tie_off;
+        } else {
+          n = 0;
+        }
+      } else {
+        n = 0;
+      }
+    }
+    for (uint64_t pend = __ballot(n != 0); pend; pend &= pend - 1) {
+      const uint32_t j = (uint32_t)__ffsll((long long)pend) - 1;
+      const uint32_t nj = readlane(n, j), oj = readlane(o, j), sj = readlane(src, j);
+      for (uint32_t i = lane; i < nj; i += 64) top[oj + i] = ties[sj + i];
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) ctr->top_used = off[n_reads];
