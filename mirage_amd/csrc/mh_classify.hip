@@ -3256,10 +3256,10 @@ __global__ void k_max_len(Reads rd, unsigned long long *out) {
 // them at any length (about 66 KB each for 32 kb reads).
 constexpr uint32_t kRefSlots = 16;
 constexpr uint32_t kRefSlotMaxLen = 2048;
-// LDS tiles give the slots about 6 KB: short reads (small row bitmaps) get
-// more of them, up to 64 (a strand of a segment shared by 120 references
-// then scores them in two rounds instead of eight).
-constexpr size_t kSlotBytes = 6144;
+// LDS tiles give the slots about 4 KB (more would cost the second-chance
+// tier entries at four workgroups per CU): 20 slots sized for 1 kb reads,
+// which multi-reference scoring turns into 44 for a 150 bp read.
+constexpr size_t kSlotBytes = 4096;
 constexpr uint32_t kMaxRefSlots = 64;
 
 uint32_t lds_slots(uint32_t qcap) {
@@ -3382,9 +3382,41 @@ std::vector<mh_tile> active_tiles(const mh_handle *h, const IndexView &ix) {
   return h->tiles.empty() ? default_tiles(h->max_read_len, ix.T, ix.epp_x16) : h->tiles;
 }
 
+// The second-chance tier's entry capacity, lowered to the largest multiple of
+// 16 at which one more of its workgroups is resident per CU, when that costs
+// at most a quarter of it.  On workload 5 (150 bp reads, 35 entries per
+// probe) 1,024 -> 848 entries takes the tier from 3 to 4 workgroups per CU:
+// 33.4 -> 25.6 ms per 4M reads, with 1.1 % of its strands passed on to the
+// HBM-workspace tier (1.7 ms); 864 entries leaves 3 (59.6 ms per call).
+void fit_tier_residency(const mh_handle *h, int s, mh_tile &t) {
+  auto resident_at = [&](uint32_t ecap) {
+    mh_tile u = t;
+    u.entries_cap = ecap;
+    return resident_blocks(u.block, tile_lds(u, tile_caps(u, s, h->lds_per_block)));
+  };
+  const uint32_t n0 = resident_at(t.entries_cap);
+  uint32_t lo = ((t.entries_cap * 3 / 4) + 15) & ~15u, hi = t.entries_cap - 16;
+  if (lo > hi || resident_at(lo) <= n0) return;
+  while (lo < hi) {  // resident_at(lo) > n0; residency falls as the capacity grows
+    const uint32_t mid = ((lo + hi + 16) / 2) & ~15u;
+    if (resident_at(mid) > n0) lo = mid;
+    else hi = mid - 16;
+  }
+  t.entries_cap = lo;
+}
+
 std::vector<mh_tile> tier_tiles(const mh_handle *h, const IndexView &ix) {
   if (h->tier_tiles.size() == 2) return h->tier_tiles;
-  return default_tier_tiles(h->max_read_len, active_tiles(h, ix)[0].entries_cap);
+  const uint32_t cap = active_tiles(h, ix)[0].entries_cap;
+  std::lock_guard<std::mutex> lk(h->tier_fit_mu);
+  if (h->tier_fit.size() != 2 || h->tier_fit_cap != cap || h->tier_fit_s != (uint32_t)ix.s) {
+    std::vector<mh_tile> t = default_tier_tiles(h->max_read_len, cap);
+    fit_tier_residency(h, ix.s, t[0]);
+    h->tier_fit = t;
+    h->tier_fit_cap = cap;
+    h->tier_fit_s = (uint32_t)ix.s;
+  }
+  return h->tier_fit;
 }
 
 // Probe records per strand of the first length tile (0: that tile stages,

@@ -203,6 +203,11 @@ struct mh_handle {
   std::vector<mh_tile> tiles;
   std::vector<mh_tile> tier_tiles;  // the two LDS overflow tiers (empty: defaults)
   uint64_t tiles_version = 0;
+  // The default overflow tiers fitted to their kernel's residency, for the
+  // first tile capacity and probe stride they were fitted for.
+  mutable std::mutex tier_fit_mu;
+  mutable std::vector<mh_tile> tier_fit;
+  mutable uint32_t tier_fit_cap = 0, tier_fit_s = 0;
   mh::Luts luts() const { return mh::Luts{d_x_log, d_smax_log, n_x, n_smax}; }
 };
 
