@@ -3571,6 +3571,15 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
     // Probe records of the first length tile's strands, also read by the LDS
     // overflow tiers for the strands that tile passes on (none without
     // MH_CLASSIFY_PREPROBE).
+    // A strand needing more entries than the second-chance tier holds goes
+    // flagged to the HBM-workspace tier.  Its 1,024 workgroups (four per
+    // CU) finish such strands sooner than the middle LDS tier, whose 32 kb
+    // query and ~3,200-entry workspace leave one workgroup per CU (2M
+    // mixed-length reads: tiers 10.5 -> 9.2 ms; workload 5 in the plain form,
+    // 2M reads: 42.9 -> 39 ms).  The middle tier keeps strands whose entries
+    // fit the second-chance tier but whose read is longer than its 1,024
+    // bases.
+    c.lds_ecap_max = tile_caps(tier_tiles(h, a.ix)[0], a.ix.s, h->lds_per_block).ecap;
     Ctx crec = c;
     if (!batched) {
       // Plain form: one launch per batch, one workgroup per query, with the
@@ -3596,14 +3605,6 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
         x.timed_buckets = 1;
       }
     } else {
-      // A strand needing more entries than the second-chance tier holds goes
-      // flagged to the HBM-workspace tier.  Its 1,024 workgroups (four per
-      // CU) finish such strands sooner than the middle LDS tier, whose
-      // 32 kb query and ~3,200-entry workspace leave one workgroup per CU.
-      // On 2M mixed-length reads this takes the overflow tiers from 10.5 to
-      // 9.2 ms.  The middle tier keeps strands whose entries fit the
-      // second-chance tier but whose read is longer than its 1,024 bases.
-      c.lds_ecap_max = tile_caps(tier_tiles(h, a.ix)[0], a.ix.s, h->lds_per_block).ecap;
       const uint32_t nb = (uint32_t)std::min<size_t>(tiles.size(), 8);
       Lens d_maxlen{};
       for (uint32_t i = 0; i < nb; i++) d_maxlen.v[i] = tiles[i].max_len;
