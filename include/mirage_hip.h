@@ -360,7 +360,8 @@ typedef struct {
   uint32_t block;          /* threads per query: 8, 16 or 32 (lane teams, 64 / block
                               queries per one-wavefront workgroup) or 64..1024
                               (a workgroup per query), a power of two */
-  uint32_t entries_cap;    /* LDS capacity in bucket entries (power of two, >= 32) */
+  uint32_t entries_cap;    /* LDS capacity in bucket entries (length tiles: a power of
+                              two >= 32; overflow tiers: a multiple of 16 >= 32) */
   uint32_t grid;           /* persistent workgroups (0: derive from occupancy) */
 } mh_tile;
 
@@ -377,9 +378,11 @@ mh_status mh_get_tiles(const mh_handle *h, const mh_index *idx, mh_tile *tiles, 
  * pass through before the HBM-workspace tiers: tier 0 (second chance) holds
  * reads up to tiers[0].max_len with tiers[0].entries_cap entries (a strand
  * with more entries skips both LDS tiers), tier 1 the longer reads.  Block
- * 64..1024, entries_cap a power of two >= 32, max_len 64..max_read_len.
- * NULL restores the defaults (sized from the first length tile).  Results
- * do not depend on them. */
+ * 64..1024, entries_cap a multiple of 16 >= 32, max_len 64..max_read_len.
+ * NULL restores the defaults: sized from the first length tile, tier 0's
+ * capacity then lowered (by at most a quarter) to the largest at which one
+ * more of its workgroups is resident per CU.  Results do not depend on
+ * them. */
 mh_status mh_set_overflow_tiles(mh_handle *h, const mh_tile *tiers);
 /* Times candidate tiles on the given device-resident reads (at most the
  * first 2^20) and installs the fastest (the project's autotuner): per length

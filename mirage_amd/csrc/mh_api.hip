@@ -579,10 +579,10 @@ mh_status mh_set_overflow_tiles(mh_handle *h, const mh_tile *tiers) {
     const mh_tile &t = tiers[i];
     const uint32_t b = t.block;
     if ((b != 64 && b != 128 && b != 256 && b != 512 && b != 1024) || t.entries_cap < 32 ||
-        (t.entries_cap & (t.entries_cap - 1)) || t.max_len < 64 || t.max_len > h->max_read_len)
+        (t.entries_cap & 15) || t.max_len < 64 || t.max_len > h->max_read_len)
       return set_error(h, MH_ERR_ARG,
                        "mh_set_overflow_tiles: two tiles; block 64..1024 (a power of two), "
-                       "entries_cap a power of two >= 32, max_len 64..max_read_len");
+                       "entries_cap a multiple of 16 >= 32, max_len 64..max_read_len");
   }
   std::lock_guard<std::mutex> lk(h->mu);
   h->tier_tiles.assign(tiers, tiers + 2);

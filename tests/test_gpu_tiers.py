@@ -185,6 +185,29 @@ def test_invalid_tile_tables_are_refused(handle):
     ix.free()
 
 
+def test_default_tiers_set_back_and_refusals(handle):
+    """The default overflow tiers that mh_get_tiles reports (the second-chance
+    capacity fitted to residency, a multiple of 16) are accepted back by
+    mh_set_overflow_tiles with the same results as the oracle; capacities that
+    are not multiples of 16 are refused."""
+    w = D.workload(1, 0.05)
+    rb, ro, qb, qo, _ = D.generate(w)
+    ix = handle.build_index(rb, ro, w.threshold)
+    oout, otop, _ = O.Index(rb, ro, w.threshold).classify(qb, qo, top_cap=512)
+    try:
+        _, tiers = handle.get_tiles(ix)
+        assert all(t[2] % 16 == 0 and t[2] >= 32 for t in tiers), tiers
+        handle.set_overflow_tiles([tuple(t) for t in tiers])
+        out, top, _ = ix.classify_host(qb, qo, flags=native.MH_CLASSIFY_BATCHED)
+        compare_reads(out, top, oout, otop, "default tiers set back")
+        with pytest.raises(native.MirageError) as e:
+            handle.set_overflow_tiles([(1024, 256, 840, 0), (handle.max_read_len, 256, 4096, 0)])
+        assert e.value.status == native.MH_ERR_ARG and "mh_set_overflow_tiles" in str(e.value)
+    finally:
+        handle.set_overflow_tiles(None)
+        ix.free()
+
+
 def test_calls_of_2_30_reads_are_refused(handle):
     """Strand ids (2 * read + strand) and the overflow flag share 32 bits: a
     call of 2^30 reads is refused before the device is touched."""
