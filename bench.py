@@ -165,7 +165,8 @@ def make_requests(rng, qb, qo, n_requests):
             zip(starts, sizes)], sizes
 
 
-def serve_stream(native, ix, reqs, sizes, loads, rng, log, max_batch_reads=65536, max_inflight=3):
+def serve_stream(native, ix, reqs, sizes, loads, rng, log, max_batch_reads=65536, max_inflight=3,
+                 flags=0):
     """Saturation (all requests submitted at once) first, then the given
     fractions of it as Poisson offered loads; latency is submit -> results
     on the host, measured by the server.  The client submits from arrays
@@ -178,7 +179,7 @@ def serve_stream(native, ix, reqs, sizes, loads, rng, log, max_batch_reads=65536
 
     def run(rate):
         srv = native.Server(ix, hbm_budget=16 << 30, max_inflight=max_inflight,
-                            max_batch_reads=max_batch_reads, flags=0)
+                            max_batch_reads=max_batch_reads, flags=flags)
         ids, lat = [], []
         out = np.zeros(nmax, native.READ_RESULT_DTYPE)
         top = np.zeros(64 * nmax + 1024, np.uint32)
@@ -476,8 +477,13 @@ def headline_leg(args, torch, native, D):
     if not args.no_serve:
         rng = np.random.default_rng(w.seed)
         reqs, sizes = make_requests(rng, qb, qo, args.serve_requests)
+        # the server runs the headline's form (with the probe pass when that
+        # was the faster batched call)
+        sflags = flags & native.MH_CLASSIFY_PREPROBE
         sv = serve_stream(native, ix, reqs, sizes, (0.25, 0.5, 0.9), rng, log,
-                          max_batch_reads=args.serve_batch, max_inflight=args.serve_slots)
+                          max_batch_reads=args.serve_batch, max_inflight=args.serve_slots,
+                          flags=sflags)
+        sv["form"] = form
         sv["workload"] = (f"requests of floor(10^U(0,4)) reads (mean {sizes.mean():.0f}) drawn "
                           "from the 10M reads, Poisson arrivals; latency submit -> results on "
                           "the host")
