@@ -110,11 +110,12 @@ def _check_served(res, reqs, oout, otop, label):
         at += n
 
 
-def test_served_requests_match_oracle(handle):
+@pytest.mark.parametrize("flags", [0, native.MH_CLASSIFY_PREPROBE])
+def test_served_requests_match_oracle(handle, flags):
     """Workload 4's recipe at 0.5 Gbp (5,000 references, k = 16): 60
     requests of log-uniform size (1-4,000 reads of 100-5,000 bp) served
-    together through three slots; every request's every read equals the
-    oracle's."""
+    together through three slots, without and with the probe pass; every
+    request's every read equals the oracle's."""
     w = D.workload(4, 0.25)
     rb, ro = D.make_refs(w.refs, w.seed)
     assert ro[-1] >= 0.49e9
@@ -130,8 +131,8 @@ def test_served_requests_match_oracle(handle):
     oout, otop, _ = O.Index.for_reads(rb, ro, w.threshold, qb, qo).classify(qb, qo, top_cap=TOP)
     h = native.Handle(0, 8192)
     ix = h.build_index(rb, ro, w.threshold)
-    res = _serve(ix, reqs, hbm_budget=4 << 30, max_inflight=3, max_batch_reads=8192)
-    _check_served(res, reqs, oout, otop, "served")
+    res = _serve(ix, reqs, hbm_budget=4 << 30, max_inflight=3, max_batch_reads=8192, flags=flags)
+    _check_served(res, reqs, oout, otop, f"served, flags {flags}")
     ix.free()
     h.close()
 
