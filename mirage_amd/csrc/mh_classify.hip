@@ -392,7 +392,8 @@ __device__ __forceinline__ bool pair_less(uint64_t ka, uint32_t va, uint64_t kb,
   return ka < kb || (ka == kb && va < vb);
 }
 
-// Ascending sort of n (key, val) pairs, any n.  Up to one wavefront's worth
+// Ascending sort of n (key, val) pairs, any n.  Whole-workgroup teams sort in
+// registers (reg_sort, defined below).  Otherwise up to one wavefront's worth
 // of elements (64, or the team's lanes) are ranked through shuffles: each
 // lane counts the smaller elements and writes its element to its rank.
 // Larger arrays use a bitonic network in LDS whose comparators all put the
@@ -663,10 +664,11 @@ __device__ __noinline__ void bitonic_sort_in(K64 *key, V32 *val, uint32_t n) {
 }
 
 
-// Sorts n distinct u64 keys in place: the rank of each key among the first
-// n by shuffles for one worker's worth; in registers (reg_sort_keys) for a
-// whole workgroup with up to 4 keys a thread; otherwise the network of
-// bitonic_sort without a value array.
+// Sorts n distinct u64 keys in place: whole-workgroup teams in registers
+// (reg_sort, up to 4 keys a thread; a one-wavefront team from 65 keys);
+// otherwise the rank of each key among the first n by shuffles for one
+// worker's worth, and the network of bitonic_sort without a value array
+// beyond that.
 template <class TM, class K64>
 __device__ __noinline__ void bitonic_sort_keys_in(K64 *key, uint32_t n) {
   if constexpr (TM::kPerWave == 1) {
