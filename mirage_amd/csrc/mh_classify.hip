@@ -392,36 +392,6 @@ __device__ __forceinline__ bool pair_less(uint64_t ka, uint32_t va, uint64_t kb,
   return ka < kb || (ka == kb && va < vb);
 }
 
-// Ascending sort of n (key, val) pairs, any n.  Whole-workgroup teams sort in
-// registers (reg_sort, defined below).  Otherwise up to one wavefront's worth
-// of elements (64, or the team's lanes) are ranked through shuffles: each
-// lane counts the smaller elements and writes its element to its rank.
-// Larger arrays use a bitonic network in LDS whose comparators all put the
-// smaller element first: each merge of size k compares i with its mirror
-// i ^ (k - 1), then with i ^ d for d = k/4 .. 1.  Positions n and beyond act
-// as +infinity and never move, so the comparisons that would touch them are
-// skipped and nothing is padded.
-// Synchronisation before a network step.  Thread t of a team handles pairs
-// p = t, t + kSize, ...; a wavefront's pairs P..P+63 (P a multiple of 64)
-// touch elements 2P..2P+127 in a step of stride up to 64, and in a mirror
-// step of size up to 128, so between two such steps each wavefront only
-// waits for itself (its LDS accesses execute in order); any other step, and
-// the step after it, needs the whole team.
-__device__ __forceinline__ bool wave_local_step(uint32_t size, uint32_t stride) {
-  return stride == (size >> 1) ? size <= 128 : stride <= 64;
-}
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-template <class TM>
-__device__ __forceinline__ void step_sync(bool local, bool &prev_local) {
-  if (local && prev_local) wave_sync();
-  else TM::sync();
-  prev_local = local;
-}
-
 // Elements of the register networks: a u64 key, or a (u64 key, u32 value)
 // pair ordered by key, then value.
 struct KeyVal {
@@ -603,6 +573,36 @@ __device__ __forceinline__ bool reg_sort_pow2(const A &arr, uint32_t n, uint32_t
 template <class TM>
 constexpr int team_log2() {
   return TM::kSize == 64 ? 6 : TM::kSize == 128 ? 7 : TM::kSize == 256 ? 8 : TM::kSize == 512 ? 9 : 10;
+}
+
+// Ascending sort of n (key, val) pairs, any n.  Whole-workgroup teams sort in
+// registers (reg_sort above).  Otherwise up to one wavefront's worth
+// of elements (64, or the team's lanes) are ranked through shuffles: each
+// lane counts the smaller elements and writes its element to its rank.
+// Larger arrays use a bitonic network in LDS whose comparators all put the
+// smaller element first: each merge of size k compares i with its mirror
+// i ^ (k - 1), then with i ^ d for d = k/4 .. 1.  Positions n and beyond act
+// as +infinity and never move, so the comparisons that would touch them are
+// skipped and nothing is padded.
+// Synchronisation before a network step.  Thread t of a team handles pairs
+// p = t, t + kSize, ...; a wavefront's pairs P..P+63 (P a multiple of 64)
+// touch elements 2P..2P+127 in a step of stride up to 64, and in a mirror
+// step of size up to 128, so between two such steps each wavefront only
+// waits for itself (its LDS accesses execute in order); any other step, and
+// the step after it, needs the whole team.
+__device__ __forceinline__ bool wave_local_step(uint32_t size, uint32_t stride) {
+  return stride == (size >> 1) ? size <= 128 : stride <= 64;
+}
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <class TM>
+__device__ __forceinline__ void step_sync(bool local, bool &prev_local) {
+  if (local && prev_local) wave_sync();
+  else TM::sync();
+  prev_local = local;
 }
 
 template <class TM, class K64, class V32>
