@@ -111,7 +111,9 @@ def kernel_table(timings, kcounters, tiles, tiers, read_len):
     """Per kernel: mean time per call (HIP events on the library stream), the
     strands it finished and their algorithmic bytes: a 16-byte directory
     pair per probe, 8 bytes per bucket entry read, 2 bytes per extension
-    comparison, the read's bases and a 32-byte result per strand."""
+    comparison, the read's bases and a 32-byte result per strand.
+    `read_len` is the bases per strand: one number, or one per row (the 8
+    length tiles, then the 4 overflow tiers)."""
     rows = []
     names = kernel_names(tiles)
     tier_kernels = [f"k_classify<BlockTeam<{tiers[0][1]}>> (second chance)",
@@ -127,8 +129,9 @@ def kernel_table(timings, kcounters, tiles, tiers, read_len):
             ms = float(np.mean([t["tier_ms"][k - 8] for t in timings]))
             name = tier_kernels[k - 8]
         c = kcounters[k]
+        rl = read_len[k] if isinstance(read_len, (list, tuple)) else read_len
         alg = (16 * c["probes"] + 8 * c["entries"] + 2 * c["ext_compares"] +
-               (read_len + 32) * c["queries"])
+               (rl + 32) * c["queries"])
         rows.append({"kernel": name, "ms": round(ms, 4), "strands": c["queries"],
                      "probes": c["probes"], "entries": c["entries"], "cands": c["cands"],
                      "ext_compares": c["ext_compares"], "alg_bytes": int(alg),
@@ -584,13 +587,27 @@ def workload2_leg(args, torch, native, D):
                                             1, native)
         ps, _, _, _, po, pt = run_steps(torch, handle, ix, md_reads, md_off, nm, 0,
                                         max(2, args.steps // 2), 1, native)
-        mrows = kernel_table(btim, bk, tiles, tiers, int(mqo[-1]) // nm)
+        # bases per strand: the mean length of each length tile's reads (the
+        # batch mean for the overflow tiers)
+        lens = np.diff(mqo.astype(np.int64))
+        mean = int(mqo[-1]) // nm
+        per_row, lo = [], 0
+        for i in range(8):
+            if i < len(tiles):
+                sel = lens[(lens > lo) & (lens <= tiles[i][0])]
+                per_row.append(int(sel.mean()) if len(sel) else mean)
+                lo = tiles[i][0]
+            else:
+                per_row.append(mean)
+        per_row += [mean] * 4
+        mrows = kernel_table(btim, bk, tiles, tiers, per_row)
         mdom = max(mrows, key=lambda x: x["ms"])
+        mtraffic, msrc = profiled_traffic("k_classify_mixed_lengths_traffic.json", mdom["kernel"])
         mixed = {"reads": nm, "bases": int(mqo[-1]), "batched_reads_per_s": round(nm / bs, 1),
                  "plain_reads_per_s": round(nm / ps, 1), "speedup": round(ps / bs, 3),
                  "same_results": same_results(native, bo, bt, po, pt),
-                 "roofline": roofline_of(mdom),
-                 "roofline_note": "read bases per strand taken as the batch mean"}
+                 "roofline": roofline_of(mdom, mtraffic, msrc),
+                 "roofline_note": "read bases per strand: the mean length of the tile's reads"}
         log(f"mixed-length: batched {nm / bs / 1e6:.3f} M reads/s, plain {nm / ps / 1e6:.3f} "
             f"M reads/s, x{ps / bs:.2f}")
         del md_reads, md_off
