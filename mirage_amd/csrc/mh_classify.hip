@@ -1978,7 +1978,13 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
   const bool use_rec = c.rec != nullptr && m <= c.rec_maxlen;
   const uint64_t *rec = use_rec ? c.rec + (uint64_t)q * c.rec_stride : nullptr;
   bool staged = false, bad;
+  // With records: this thread's probe record of the first pass, loaded with
+  // the status word (one memory round trip); a record row holds every probe
+  // of a read of up to rec_maxlen bases.
+  uint64_t rec_first = kRecAbsent;
   if (use_rec) {
+    const uint32_t np0 = m > (uint32_t)T ? (m - T + s - 1) / s : 0u;
+    if (tid < np0 && tid + 1 < c.rec_stride) rec_first = rec[1 + tid];
     bad = rec[0] == kRecBad;
   } else {
     bad = stage();
@@ -2013,7 +2019,7 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
     uint64_t lo = 0;
     if (p < nprobe) {
       if (use_rec) {
-        const uint64_t r = rec[1 + p];
+        const uint64_t r = base == 0 ? rec_first : rec[1 + p];
         present = r != kRecAbsent;
         if (present) {
           lo = r >> 24;
