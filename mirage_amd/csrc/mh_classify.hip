@@ -1194,28 +1194,34 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
   for (uint32_t base = 0; base < nranks; base += BS) {
     const uint32_t r = base + tid;
     const bool f = r < nranks && ws.t3[r] == G;
-    uint32_t tot;
-    const uint32_t idx = TM::excl_scan(f ? 1u : 0u, ws.red, tot);
-    // This thread's reference, where its bytes lie: loaded once, before the
+    // This thread's reference, where its bytes lie (loaded once, before the
     // rounds, so that the loads of every round's references are in flight
-    // together.
-    uint32_t R = 0;
+    // together), and its seeds: its chains, and for reference 0 the unstored
+    // copies of (0, 0) as one-pair seeds, one per probe that hit bucket 0
+    // (Analyser.cpp:2421-2481).
+    uint32_t R = 0, c0 = 0, nchain = 0, nseed = 0;
     uint64_t ro = 0, rlen = 0;
     if (f) {
       R = ws.rref[r];
       ro = ix.ref_off[R];
       rlen = ix.ref_off[R + 1] - ro;
+      c0 = ws.v1[r];
+      nchain = ((r + 1 < nranks) ? ws.v1[r + 1] : nchains) - c0;
+      nseed = nchain + (R == 0 ? nph : 0u);
     }
+    // One scan numbers the candidates (bits 21 and up) and the seeds before
+    // each (bits 0-20: a team's seeds stay below 2^21, its candidates below
+    // 2^11).  When they all fit one round, each slot's first item is its
+    // seed offset from this scan.
+    uint32_t sums;
+    const uint32_t ex = TM::excl_scan(f ? (1u << 21) | nseed : 0u, ws.red, sums);
+    const uint32_t idx = ex >> 21, tot = sums >> 21;
+    const bool one_round = tot <= nslot;
     for (uint32_t g = 0; g < tot; g += nslot) {
       const uint32_t nb = min(nslot, tot - g);
       TM::sync();
       if (f && idx >= g && idx < g + nb) {
         RefSlot &sl = ws.slots[idx - g];
-        const uint32_t c0 = ws.v1[r];
-        const uint32_t c1 = (r + 1 < nranks) ? ws.v1[r + 1] : nchains;
-        // reference 0 also gets the unstored copies of (0, 0) as one-pair
-        // seeds, one per probe that hit bucket 0 (Analyser.cpp:2421-2481)
-        const uint32_t nvirt = R == 0 ? nph : 0u;
         sl.acc[0] = 0;
         sl.acc[1] = 0;
         sl.ro = ro;
@@ -1223,14 +1229,20 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
         sl.r = r;
         sl.R = R;
         sl.c0 = c0;
-        sl.nchain = c1 - c0;
-        sl.nseed = c1 - c0 + nvirt;
+        sl.nchain = nchain;
+        sl.nseed = nseed;
+        if (one_round) {
+          sl.base = ex & 0x1FFFFFu;
+          seeds_part += nchain + (unsigned long long)(nseed - nchain) * ix.phantom;
+        }
       }
       for (uint32_t w = tid; w < nb * bw; w += BS) slot_bm[w] = 0;
       TM::sync();
-      // each slot's first item: a scan of the seed counts (nb <= BS)
       uint32_t nitems;
-      {
+      if (one_round) {
+        nitems = sums & 0x1FFFFFu;
+      } else {
+        // each slot's first item: a scan of the round's seed counts (nb <= BS)
         const uint32_t ns = tid < nb ? ws.slots[tid].nseed : 0u;
         const uint32_t base_t = TM::excl_scan(ns, ws.red, nitems);
         if (tid < nb) {
@@ -1239,9 +1251,9 @@ __device__ __forceinline__ void score_multi(const Ctx &c, const Ws &ws, const Ca
                         (unsigned long long)(ws.slots[tid].nseed - ws.slots[tid].nchain) *
                             ix.phantom;
         }
-        seeded += nb;
+        TM::sync();
       }
-      TM::sync();
+      seeded += nb;
       unsigned long long ncmp = 0;
       for (uint32_t i = wid; i < nitems; i += NW) {
         uint32_t b = 0, hi = nb;  // the slot of item i: the last with base <= i
@@ -2443,7 +2455,7 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
   unsigned long long seeds_total = 0;
   uint32_t seeded = 0;
   bool multi = false;
-  if constexpr (TM::kPerWave == 1) {
+  if constexpr (TM::kPerWave == 1 && BS >= 128) {  // slots need 128 threads or more
     if (cap.nslot != 0 && nranks >= 2) {
       multi = true;
       uint32_t lut_bad = 0;
