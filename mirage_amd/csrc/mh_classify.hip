@@ -905,6 +905,18 @@ __device__ inline Ws ws_carve(unsigned char *base, const Caps &c) {
   return w;
 }
 
+// A workspace in HBM comes with an LDS sort buffer (ws.stage).  Only the
+// HBM-workspace kernels' teams (256 and 1,024 threads) can have one, so the
+// other teams compile without the staged paths (every test of the buffer
+// goes through here, so a team always takes one form throughout).
+template <class TM>
+__device__ __forceinline__ bool has_stage(const Ws &ws) {
+  if constexpr (TM::kPerWave == 1 && (TM::kSize == 256 || TM::kSize == 1024))
+    return ws.stage != nullptr;
+  else
+    return false;
+}
+
 struct Ctx {
   IndexView ix;
   Luts lut;
@@ -1498,7 +1510,7 @@ __device__ void filter_hashed(const Ws &ws, uint32_t E, uint32_t ecap, uint32_t 
   TM::sync();
   uint32_t maxrun = 0;
   if (nm) {
-    if (ws.stage)
+    if (has_stage<TM>(ws))
       staged_sort_keys<TM>(ws.k2, nm, ws.stage, ws.stage_cap);
     else
       bitonic_sort<TM>(ws.k2, ws.t3, nm);  // t3: value scratch, unused
@@ -1517,14 +1529,14 @@ __device__ void filter_hashed(const Ws &ws, uint32_t E, uint32_t ecap, uint32_t 
     if (e < E) {
       key = ws.k1[e];
       pos = ws.v1[e];
-      if (!ws.stage) j = ws.hit_j[(uint32_t)key];
+      if (!has_stage<TM>(ws)) j = ws.hit_j[(uint32_t)key];
       keep = rval[ws.v2[e]] >= cut ? 1u : 0u;
     }
     uint32_t tot;
     const uint32_t ex = TM::excl_scan(keep, ws.red, tot);
     TM::sync();
     if (keep) {
-      if (ws.stage) {
+      if (has_stage<TM>(ws)) {
         // packed for a key-only sort: ref << 44 | refPos << 12 | hit
         ws.k2[nkeep + ex] = ((key >> 32) << 44) | ((uint64_t)pos << 12) | (uint32_t)key;
       } else {
@@ -2276,7 +2288,7 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
         }
         // Sort by (reference, refPos, qPos) (Analyser.cpp:1024-1059): the
         // packed keys order so (hit order is qPos order); then unpack.
-        if (ws.stage)
+        if (has_stage<TM>(ws))
           staged_sort_keys<TM>(ws.k2, nP, ws.stage, ws.stage_cap);
         else
           bitonic_sort_keys<TM>(ws.k2, nP);
@@ -2294,7 +2306,7 @@ __device__ __forceinline__ uint32_t process_query(const Ctx &c, const Ws &ws, co
           filter_sorted<TM>(ws, E, nP);
         // Sort by (reference, refPos, qPos) (Analyser.cpp:1024-1059).
         TM::sync();
-        if (hashed && ws.stage) {
+        if (hashed && has_stage<TM>(ws)) {
           // filter_hashed packed the pairs as ref << 44 | refPos << 12 | hit
           staged_sort_keys<TM>(ws.k2, nP, ws.stage, ws.stage_cap);
           for (uint32_t t = tid; t < nP; t += BS) {
