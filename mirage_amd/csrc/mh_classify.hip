@@ -3677,6 +3677,9 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
       // Long buckets first so that their tail overlaps with the short ones.
       if (timed) hipEventRecord(x.ev[9], s);
       for (int b = (int)nb - 1; b >= 0; b--) {
+        // A read longer than tiles[b - 1].max_len is the only kind that bins
+        // into tile b or later.
+        if (b > 0 && a.max_len_hint <= tiles[b - 1].max_len) continue;
         const mh_tile &t = tiles[b];
         const Caps cap = tile_caps(t, a.ix.s, h->lds_per_block);
         uint32_t grid = t.grid;

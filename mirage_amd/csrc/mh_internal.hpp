@@ -240,6 +240,10 @@ struct ClassifyArgs {
   uint32_t flags;
   Outputs out;
   mh_read_result *final_out;
+  // Longest read of the call when the host knows it (the server builds its
+  // batches on the host): the batched form then skips the length tiles that
+  // no read can reach.  UINT64_MAX: unknown, every tile is launched.
+  uint64_t max_len_hint = UINT64_MAX;
 };
 // A scratch region of fixed size that a classify call uses instead of the
 // execution context's arena (nothing is allocated): the HBM tiers run on as

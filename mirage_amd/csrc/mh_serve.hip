@@ -123,12 +123,15 @@ mh_status init_slot(mh_handle *h, Slot &s, uint64_t cap_bytes, uint64_t cap_read
 // the slot is busy and its batch's requests are no longer pending.
 mh_status launch_slot(mh_server *sv, Slot &s) {
   mh_handle *h = sv->h;
-  uint64_t pos = 0;
+  uint64_t pos = 0, longest = 0;
   uint32_t r = 0;
   s.h_off[0] = 0;
   for (Request *q : s.batch) {
     std::memcpy(s.h_reads + pos, q->reads.data(), q->reads.size());
-    for (uint32_t i = 0; i < q->n; i++) s.h_off[r + i + 1] = pos + q->off[i + 1];
+    for (uint32_t i = 0; i < q->n; i++) {
+      s.h_off[r + i + 1] = pos + q->off[i + 1];
+      longest = std::max<uint64_t>(longest, q->off[i + 1] - q->off[i]);
+    }
     pos += q->reads.size();
     r += q->n;
   }
@@ -145,6 +148,7 @@ mh_status launch_slot(mh_server *sv, Slot &s) {
   a.out.top = s.d_top;
   a.out.top_cap = s.top_cap;
   a.final_out = s.d_out;
+  a.max_len_hint = longest;
   mh_status stt;
   {
     std::lock_guard<std::mutex> lk(h->mu);

@@ -61,6 +61,31 @@ def test_server_admits_while_busy(served):
     assert admits[-1] > admits[0]  # not all in one batch
 
 
+def test_served_reads_at_tile_boundaries(served, handle):
+    """The server launches only the length tiles its batch's longest read
+    can reach: single-read batches of every tile's maximum length and one
+    base more give the results of a batched call, which launches every
+    tile."""
+    w, ix, qb, qo = served
+    tiles = handle.get_tiles(ix)[0]
+    lens = np.diff(qo)
+    src = int(np.argmax(lens))
+    a, longest = int(qo[src]), int(lens[src])
+    cut = sorted({m + d for t in tiles[:-1] for m in [t[0]] for d in (0, 1)
+                  if 0 < m + d <= longest} | {1, longest})
+    assert len(cut) >= 4
+    reqs = [(qb[a:a + n].copy(), np.array([0, n], dtype=qo.dtype)) for n in cut]
+    srv = native.Server(ix, max_inflight=1, max_batch_reads=1)
+    for b, o in reqs:
+        aout, atop, _ = ix.classify_host(b, o, flags=native.MH_CLASSIFY_BATCHED)
+        out, top, info = srv.wait(srv.submit(b, o))
+        assert info["n_reads"] == 1
+        assert np.array_equal(out["highest"].view(np.uint32), aout["highest"].view(np.uint32)), len(b)
+        assert np.array_equal(out["strand"], aout["strand"]), len(b)
+        assert native.tie_sets(out, top) == native.tie_sets(aout, atop), len(b)
+    srv.stop()
+
+
 def test_autotune_keeps_results(handle):
     import torch
     w = D.workload(3, 0.001)
