@@ -3618,6 +3618,10 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
     // bases.
     c.lds_ecap_max = tile_caps(tier_tiles(h, a.ix)[0], a.ix.s, h->lds_per_block).ecap;
     Ctx crec = c;
+    // The persistent kernels take strands from queues, so a grid of more
+    // workgroups than the call has strands would only dispatch idle ones.
+    const uint32_t max_grid = (uint32_t)std::min<uint64_t>(nq, 1u << 30);
+    const uint32_t hbm_grid = std::min(hbm_blocks, max_grid);
     if (!batched) {
       // Plain form: one launch per batch, one workgroup per query, with the
       // single tile that covers the longest read of the batch.
@@ -3689,6 +3693,7 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
         const Caps cap = tile_caps(t, a.ix.s, h->lds_per_block);
         uint32_t grid = t.grid;
         if (!grid) grid = h->cu_count * resident_blocks(t.block, tile_lds(t, cap));
+        grid = std::min(grid, max_grid);
         // queue b: binned[start[b], start[b] + cnt[b]), head queue_head[b]
         Ctx cb = b == 0 ? c0 : c;
         cb.kid = (uint32_t)b;
@@ -3716,8 +3721,8 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
       ct.kid = kTierRow0 + (uint32_t)t;
       ct.out.overflow = list[(t + 1) & 1];
       ct.out.overflow_cnt = &d->tier_n[t + 1];
-      const uint32_t grid =
-          h->cu_count * resident_blocks(lds_tiers[t].block, tile_lds(lds_tiers[t], tc));
+      const uint32_t grid = std::min(
+          max_grid, h->cu_count * resident_blocks(lds_tiers[t].block, tile_lds(lds_tiers[t], tc)));
       e = launch_tile(lds_tiers[t].block, ct, tc, list[t & 1], nullptr, 0, &d->tier_n[t],
                       &d->tier_head[t], grid, s);
       if (e != hipSuccess) return hip_error(h, e, "k_classify overflow tier");
@@ -3730,18 +3735,19 @@ mh_status launch_classify(mh_handle *h, Exec &x, const ClassifyArgs &a, bool bat
       ch.out.overflow = list[1];
       ch.out.overflow_cnt = &d->tier_n[3];
       if (n <= kHbmWideCall)
-        hipLaunchKernelGGL(k_classify_hbm<1024>, dim3(hbm_blocks), dim3(1024), 0, s, ch, ca,
+        hipLaunchKernelGGL(k_classify_hbm<1024>, dim3(hbm_grid), dim3(1024), 0, s, ch, ca,
                            big_ws, ws_stride(ca), (const uint32_t *)list[0],
                            (const unsigned long long *)&d->tier_n[2], &d->tier_head[2], 0);
       else
-        hipLaunchKernelGGL(k_classify_hbm<256>, dim3(hbm_blocks), dim3(256), 0, s, ch, ca,
+        hipLaunchKernelGGL(k_classify_hbm<256>, dim3(hbm_grid), dim3(256), 0, s, ch, ca,
                            big_ws, ws_stride(ca), (const uint32_t *)list[0],
                            (const unsigned long long *)&d->tier_n[2], &d->tier_head[2], 0);
       if (timed) hipEventRecord(x.ev[15], s);
       const Caps cb = hbm_caps(h->max_read_len, a.ix.s, kBigEcap);
       Ctx cl = c;
       cl.kid = kTierRow0 + 3;
-      hipLaunchKernelGGL(k_classify_hbm<1024>, dim3(big_blocks), dim3(1024), 0, s, cl, cb, big_ws,
+      hipLaunchKernelGGL(k_classify_hbm<1024>, dim3(std::min(big_blocks, max_grid)), dim3(1024), 0,
+                         s, cl, cb, big_ws,
                          ws_stride(cb), (const uint32_t *)list[1],
                          (const unsigned long long *)&d->tier_n[3], &d->tier_head[3], 1);
     }
